@@ -1,0 +1,175 @@
+/*
+ * mgsort.h — C ABI of the MI355X (gfx950) sorting library libmgsort.so.
+ *
+ * This is the drop-in boundary that replaces the device primitives the
+ * reference calls through thrust/CUB/moderngpu and its own SelectPivot kernel.
+ * Every entry point is stream-ordered and asynchronous: it enqueues HIP kernels
+ * on `stream` (a hipStream_t passed as void*, NULL = the legacy default stream)
+ * and returns.  The caller selects the device with hipSetDevice() beforehand
+ * and owns every buffer; the library never allocates device memory on these
+ * paths — scratch comes from the caller's `temp` buffer, whose size is given by
+ * the matching *_temp_bytes() query (the two-call pattern of
+ * cub::DeviceRadixSort::SortKeys, reference
+ * src/local_gpu_sort_algorithm_benchmark.cu:53-65).
+ *
+ * Key order ("encoded order").  Keys are ordered by an unsigned image enc(x):
+ *   unsigned: x;  signed: x with the sign bit flipped;
+ *   float/double: bits | sign-bit if the sign is clear, ~bits if it is set,
+ *   after mapping -0.0 onto +0.0.
+ * Sorts are stable ascending in enc order.  For NaN-free input this is the
+ * order of std::stable_sort(less) and of thrust::sort on arithmetic keys
+ * (reference src/tanasic_sort.cuh:264-266); -NaN sorts first, +NaN last,
+ * -0.0/+0.0 keep their input order, denormals sort by value.
+ *
+ * Status codes: 0 on success, a negative MGS_ERR_* otherwise (no exceptions,
+ * no exit(); mgs_status_str() gives the text).  Unlike the reference, whose
+ * CheckCudaError is a no-op in release builds (src/cuda_error.cuh:5-10), every
+ * HIP launch is checked.
+ */
+#ifndef MGSORT_H_
+#define MGSORT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Key types (the reference instantiates int, long, float, double:
+ * src/sort_benchmark.cu:209-221; u32/u64 are added for the MI355X configs). */
+typedef enum mgs_key {
+  MGS_KEY_U32 = 0,
+  MGS_KEY_U64 = 1,
+  MGS_KEY_I32 = 2,
+  MGS_KEY_I64 = 3,
+  MGS_KEY_F32 = 4,
+  MGS_KEY_F64 = 5
+} mgs_key;
+
+/* Payload types for key-value sorts (none in the reference). */
+typedef enum mgs_val {
+  MGS_VAL_NONE = 0,
+  MGS_VAL_U32 = 1,
+  MGS_VAL_U64 = 2
+} mgs_val;
+
+enum {
+  MGS_OK = 0,
+  MGS_ERR_BAD_TYPE = -1,       /* unknown mgs_key / mgs_val                   */
+  MGS_ERR_TEMP_TOO_SMALL = -2, /* temp_bytes below the *_temp_bytes() answer  */
+  MGS_ERR_HIP = -3,            /* a HIP call failed; see mgs_last_hip_error() */
+  MGS_ERR_INVALID = -4,        /* bad argument (NULL pointer, size, overlap)  */
+  MGS_ERR_TOO_LARGE = -5       /* n beyond what the kernels index (2^32 - 1)  */
+};
+
+/* Tuning of the radix kernels; a zeroed struct (or NULL) selects defaults
+ * from the built-in table chosen by measurement on MI355X. */
+typedef struct mgs_tuning {
+  int32_t tile_keys;     /* keys per onesweep tile (0 = default)            */
+  int32_t small_max;     /* segments up to this length sort inside one      */
+                         /* workgroup's LDS (0 = default)                   */
+  int32_t medium_max;    /* segments up to this length sort with a single   */
+                         /* workgroup streaming HBM (0 = default)           */
+  int32_t reserved;
+} mgs_tuning;
+
+/* --------------------------------------------------------------------------
+ * Radix sort.  Replaces thrust::sort(par(alloc).on(stream), b, e) at
+ * reference src/tanasic_sort.cuh:264-266, src/gowanlock_sort.cuh:188-190 and
+ * :337-340, src/sort_benchmark.cu:89, and cub::DeviceRadixSort::SortKeys at
+ * src/local_gpu_sort_algorithm_benchmark.cu:55-65.
+ *
+ * Sorts n keys (and, if vt != MGS_VAL_NONE, their payloads) stably by enc().
+ * keys_in may equal keys_out (in place, thrust::sort semantics) and vals_in
+ * may equal vals_out; otherwise the inputs are only read.  n <= 2^32 - 1.
+ * ------------------------------------------------------------------------ */
+size_t mgs_sort_temp_bytes(mgs_key kt, mgs_val vt, size_t n);
+
+int mgs_sort(mgs_key kt, mgs_val vt,
+             const void* keys_in, void* keys_out,
+             const void* vals_in, void* vals_out,
+             size_t n, void* temp, size_t temp_bytes, void* stream);
+
+/* --------------------------------------------------------------------------
+ * Merge-path merge.  Replaces thrust::merge at reference
+ * src/tanasic_sort.cuh:163-167 and src/local_gpu_merge_algorithm_benchmark.cu
+ * :157-158 (and mgpu::merge at :165-167).  Stable: on equal keys the element
+ * of A comes first.  Output must not overlap the inputs.
+ * ------------------------------------------------------------------------ */
+size_t mgs_merge_temp_bytes(mgs_key kt, mgs_val vt, size_t na, size_t nb);
+
+int mgs_merge(mgs_key kt, mgs_val vt,
+              const void* a_keys, const void* a_vals, size_t na,
+              const void* b_keys, const void* b_vals, size_t nb,
+              void* out_keys, void* out_vals,
+              void* temp, size_t temp_bytes, void* stream);
+
+/* --------------------------------------------------------------------------
+ * Tanasic pivot.  Replaces SelectPivot/FindPivot at reference
+ * src/tanasic_sort.cuh:15-87.  left_parts/right_parts are HOST arrays of
+ * num_parts DEVICE pointers (num_parts <= 8), each partition part_size keys
+ * sorted in enc order; L and R are their concatenations.  Writes
+ *   p = max k in [0, M] with k == 0 or enc(L[M-k]) > enc(R[k-1]),
+ *   M = part_size * num_parts,
+ * as a uint64 to *result, which may be device memory or mapped pinned host
+ * memory (as the reference's HostAllocator slot, src/host_allocator.cuh:11-12).
+ * ------------------------------------------------------------------------ */
+int mgs_select_pivot(mgs_key kt,
+                     const void* const* left_parts, const void* const* right_parts,
+                     size_t part_size, size_t num_parts,
+                     uint64_t* result, void* stream);
+
+/* --------------------------------------------------------------------------
+ * Range swap.  Replaces the pairwise exchange of SwapPartitions at reference
+ * src/tanasic_sort.cuh:103-143 for two partitions on one device: exchanges
+ * a[0..n) with b[0..n) in place (element order kept).  The ranges must not
+ * overlap.  a_vals/b_vals are ignored when vt == MGS_VAL_NONE.
+ * ------------------------------------------------------------------------ */
+int mgs_swap_ranges(mgs_key kt, mgs_val vt,
+                    void* a_keys, void* a_vals, void* b_keys, void* b_vals,
+                    size_t n, void* stream);
+
+/* --------------------------------------------------------------------------
+ * Filler.  Replaces thrust::fill(..., numeric_limits<T>::max()) at reference
+ * src/tanasic_sort.cuh:256-259 and src/gowanlock_sort.cuh:180-183,329-332.
+ * Writes the key whose enc() is all ones (for floats a +NaN, which sorts
+ * after +inf, where the reference's FLT_MAX would not), so fillers always
+ * sort last.
+ * ------------------------------------------------------------------------ */
+int mgs_fill_max(mgs_key kt, void* keys, size_t n, void* stream);
+
+/* --------------------------------------------------------------------------
+ * Segmented sort (no reference counterpart: the batching layer).  Sorts
+ * nseg independent segments stably by enc(); segment s is
+ * [offsets[s], offsets[s+1]) of keys/vals.  d_offsets is a DEVICE array of
+ * nseg+1 uint64 offsets, non-decreasing, offsets[0] may be > 0.
+ * keys_in may equal keys_out.  `tuning` may be NULL.  One call enqueues a
+ * fixed launch sequence independent of the segment lengths, so it can be
+ * captured once in a hipGraph and replayed.  max_total_keys bounds
+ * offsets[nseg] - offsets[0] and sizes the scratch.
+ * ------------------------------------------------------------------------ */
+size_t mgs_segmented_sort_temp_bytes(mgs_key kt, mgs_val vt,
+                                     size_t max_total_keys, size_t nseg);
+
+int mgs_segmented_sort(mgs_key kt, mgs_val vt,
+                       const void* keys_in, void* keys_out,
+                       const void* vals_in, void* vals_out,
+                       const uint64_t* d_offsets, size_t nseg,
+                       size_t max_total_keys, const mgs_tuning* tuning,
+                       void* temp, size_t temp_bytes, void* stream);
+
+/* Status text; never NULL. */
+const char* mgs_status_str(int status);
+
+/* The hipError_t of the last MGS_ERR_HIP on this thread (0 if none). */
+int mgs_last_hip_error(void);
+
+/* Library version string. */
+const char* mgs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGSORT_H_ */

@@ -1,0 +1,107 @@
+// extern "C" entry points of libmgsort (declared in include/mgsort.h).
+#include "internal.hpp"
+
+namespace mgs {
+
+namespace {
+thread_local int g_last_hip_error = 0;
+}
+
+int record_hip(hipError_t e) {
+  if (e == hipSuccess) return MGS_OK;
+  g_last_hip_error = (int)e;
+  return MGS_ERR_HIP;
+}
+
+int last_hip_error() { return g_last_hip_error; }
+
+}  // namespace mgs
+
+using mgs::key_bytes;
+using mgs::val_bytes;
+
+namespace {
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+constexpr size_t kMaxKeys = 0xFFFFFFFFull;  // kernels index with 32 bits
+}  // namespace
+
+extern "C" {
+
+size_t mgs_sort_temp_bytes(mgs_key kt, mgs_val vt, size_t n) { return mgs::sort_temp_bytes(kt, vt, n); }
+
+int mgs_sort(mgs_key kt, mgs_val vt, const void* keys_in, void* keys_out, const void* vals_in, void* vals_out,
+             size_t n, void* temp, size_t temp_bytes, void* stream) {
+  if (key_bytes(kt) < 0 || val_bytes(vt) < 0) return MGS_ERR_BAD_TYPE;
+  if (n == 0) return MGS_OK;
+  if (n > kMaxKeys) return MGS_ERR_TOO_LARGE;
+  if (keys_in == nullptr || keys_out == nullptr) return MGS_ERR_INVALID;
+  if (vt != MGS_VAL_NONE && (vals_in == nullptr || vals_out == nullptr)) return MGS_ERR_INVALID;
+  return mgs::sort_dispatch(kt, vt, keys_in, keys_out, vals_in, vals_out, n, temp, temp_bytes, as_stream(stream));
+}
+
+size_t mgs_merge_temp_bytes(mgs_key kt, mgs_val vt, size_t na, size_t nb) {
+  return mgs::merge_temp_bytes(kt, vt, na, nb);
+}
+
+int mgs_merge(mgs_key kt, mgs_val vt, const void* a_keys, const void* a_vals, size_t na, const void* b_keys,
+              const void* b_vals, size_t nb, void* out_keys, void* out_vals, void* temp, size_t temp_bytes,
+              void* stream) {
+  if (key_bytes(kt) < 0 || val_bytes(vt) < 0) return MGS_ERR_BAD_TYPE;
+  if (na + nb == 0) return MGS_OK;
+  if ((na > 0 && a_keys == nullptr) || (nb > 0 && b_keys == nullptr) || out_keys == nullptr) return MGS_ERR_INVALID;
+  if (vt != MGS_VAL_NONE &&
+      ((na > 0 && a_vals == nullptr) || (nb > 0 && b_vals == nullptr) || out_vals == nullptr))
+    return MGS_ERR_INVALID;
+  return mgs::merge_dispatch(kt, vt, a_keys, a_vals, na, b_keys, b_vals, nb, out_keys, out_vals, temp, temp_bytes,
+                             as_stream(stream));
+}
+
+int mgs_select_pivot(mgs_key kt, const void* const* left_parts, const void* const* right_parts, size_t part_size,
+                     size_t num_parts, uint64_t* result, void* stream) {
+  if (key_bytes(kt) < 0) return MGS_ERR_BAD_TYPE;
+  return mgs::pivot_dispatch(kt, left_parts, right_parts, part_size, num_parts, result, as_stream(stream));
+}
+
+int mgs_swap_ranges(mgs_key kt, mgs_val vt, void* a_keys, void* a_vals, void* b_keys, void* b_vals, size_t n,
+                    void* stream) {
+  return mgs::swap_dispatch(kt, vt, a_keys, a_vals, b_keys, b_vals, n, as_stream(stream));
+}
+
+int mgs_fill_max(mgs_key kt, void* keys, size_t n, void* stream) {
+  if (key_bytes(kt) < 0) return MGS_ERR_BAD_TYPE;
+  return mgs::fill_max_dispatch(kt, keys, n, as_stream(stream));
+}
+
+size_t mgs_segmented_sort_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total_keys, size_t nseg) {
+  return mgs::segmented_temp_bytes(kt, vt, max_total_keys, nseg);
+}
+
+int mgs_segmented_sort(mgs_key kt, mgs_val vt, const void* keys_in, void* keys_out, const void* vals_in,
+                       void* vals_out, const uint64_t* d_offsets, size_t nseg, size_t max_total_keys,
+                       const mgs_tuning* tuning, void* temp, size_t temp_bytes, void* stream) {
+  if (key_bytes(kt) < 0 || val_bytes(vt) < 0) return MGS_ERR_BAD_TYPE;
+  if (nseg == 0 || max_total_keys == 0) return MGS_OK;
+  if (max_total_keys > kMaxKeys) return MGS_ERR_TOO_LARGE;
+  if (keys_in == nullptr || keys_out == nullptr || d_offsets == nullptr) return MGS_ERR_INVALID;
+  if (vt != MGS_VAL_NONE && (vals_in == nullptr || vals_out == nullptr)) return MGS_ERR_INVALID;
+  return mgs::segmented_dispatch(kt, vt, keys_in, keys_out, vals_in, vals_out, d_offsets, nseg, max_total_keys, tuning,
+                                 temp, temp_bytes, as_stream(stream));
+}
+
+const char* mgs_status_str(int status) {
+  switch (status) {
+    case MGS_OK: return "ok";
+    case MGS_ERR_BAD_TYPE: return "unknown key or value type";
+    case MGS_ERR_TEMP_TOO_SMALL: return "temporary buffer too small";
+    case MGS_ERR_HIP: return "HIP runtime error";
+    case MGS_ERR_INVALID: return "invalid argument";
+    case MGS_ERR_TOO_LARGE: return "too many keys";
+  }
+  return "unknown status";
+}
+
+int mgs_last_hip_error(void) { return mgs::last_hip_error(); }
+
+const char* mgs_version(void) { return "mgsort-mi355x 0.1 (gfx950)"; }
+
+}  // extern "C"

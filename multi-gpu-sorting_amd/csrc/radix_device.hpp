@@ -1,0 +1,261 @@
+// Tile-level device routines of the radix sort, shared by the plain sort
+// (radix_sort.hip) and the segmented sort (segmented.hip).
+#pragma once
+
+#include <type_traits>
+
+#include "radix_sort.hpp"
+
+namespace mgs {
+
+constexpr uint32_t kLookbackSpinLimit = 1u << 24;
+
+// Look-back over predecessor tiles of one chain for one digit: sums
+// aggregates until an inclusive prefix is found.  `desc` points at the
+// chain's first tile.
+template <typename D>
+__device__ __forceinline__ uint32_t lookback(const D* desc, uint32_t tile, uint32_t d, int parity) {
+  uint32_t prefix = 0;
+  uint32_t t = tile - 1;
+  uint32_t spins = 0;
+  for (;;) {
+    const D v = ld_agent(desc + (size_t)t * kRadix + d);
+    const uint32_t st = Desc<D>::status(v, parity);
+    if (st == kStEmpty) {
+      // Bounded wait: a predecessor tile always publishes within microseconds;
+      // the bound only keeps a corrupted descriptor array from hanging the GPU
+      // (the result is then wrong, every store stays bounds-checked).
+      if (++spins > kLookbackSpinLimit) break;
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    prefix += (uint32_t)Desc<D>::value(v);
+    if (st == kStInclusive) break;
+    --t;
+  }
+  return prefix;
+}
+
+// ---------------------------------------------------------------------------
+// One onesweep tile.
+//
+// src/dst/vsrc/vdst  the chain's arrays (index 0 = first key of the chain)
+// n                  keys in the chain
+// tile               tile index inside the chain
+// desc               the chain's descriptors (tile-major, 256 per tile)
+// digit_base         256 exclusive digit bases of the chain for this byte
+// ---------------------------------------------------------------------------
+template <int KT, typename V, int BLOCK, int IPT, typename D>
+__device__ __forceinline__ void onesweep_tile(const typename Codec<KT>::Bits* __restrict__ src,
+                                              typename Codec<KT>::Bits* __restrict__ dst, const V* __restrict__ vsrc,
+                                              V* __restrict__ vdst, uint32_t n, uint32_t tile, D* __restrict__ desc,
+                                              const uint32_t* __restrict__ digit_base, int shift, int parity) {
+  using Bits = typename Codec<KT>::Bits;
+  constexpr int W = BLOCK / kWave;
+  constexpr int TILE = BLOCK * IPT;
+  constexpr bool HV = kHasVal<V>;
+  using VS = typename std::conditional<HV, V, uint32_t>::type;
+  static_assert(W >= 4, "need >= 256 threads for the per-digit phase");
+
+  struct RankSmem {
+    uint64_t peers[W][kRadix];
+  };
+  union StageSmem {
+    RankSmem r;
+    Bits keys[TILE];
+  };
+  __shared__ StageSmem u;
+  __shared__ VS vstage[HV ? TILE : 1];
+  __shared__ uint32_t off[W][kRadix];
+  __shared__ uint32_t gbase[kRadix];
+  __shared__ uint32_t sums[4];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int w = tid / kWave;
+  for (int i = tid; i < W * kRadix; i += BLOCK) {
+    (&u.r.peers[0][0])[i] = 0;
+    (&off[0][0])[i] = 0;
+  }
+  __syncthreads();
+  const uint32_t tile_base = tile * (uint32_t)TILE;
+  const uint32_t tile_cnt = min((uint32_t)TILE, n - tile_base);
+  const bool full = tile_cnt == (uint32_t)TILE;
+
+  Bits key[IPT];
+  VS val[IPT];
+  uint32_t dg[IPT], rk[IPT];
+  const uint32_t wb = tile_base + (uint32_t)(w * IPT * kWave + lane);
+  if (full) {
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) key[k] = src[wb + k * kWave];
+    if constexpr (HV) {
+#pragma unroll
+      for (int k = 0; k < IPT; ++k) val[k] = vsrc[wb + k * kWave];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      const uint32_t idx = wb + k * kWave;
+      key[k] = idx < n ? src[idx] : Bits(0);
+      if constexpr (HV) val[k] = idx < n ? vsrc[idx] : VS(0);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) dg[k] = digit_of<KT>(key[k], shift);
+
+  uint64_t* pt = u.r.peers[w];
+  uint32_t* ct = off[w];
+  if (full) {
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) rk[k] = rank_wave(dg[k], pt, ct);
+  } else {
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      if (wb + k * kWave < n) rk[k] = rank_wave(dg[k], pt, ct);
+    }
+  }
+  __syncthreads();
+
+  // Per-digit phase: threads 0..255 own one digit each.
+  uint32_t total = 0;
+  D* my_desc = desc + (size_t)tile * kRadix + tid;
+  if (tid < kRadix) {
+#pragma unroll
+    for (int i = 0; i < W; ++i) total += off[i][tid];
+    st_agent(my_desc, Desc<D>::make(tile == 0 ? kStInclusive : kStAggregate, (D)total, parity));
+  }
+  const uint32_t excl = block_excl_scan_256(total, sums, tid);
+  if (tid < kRadix) {
+    uint32_t run = excl;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      const uint32_t c = off[i][tid];
+      off[i][tid] = run;
+      run += c;
+    }
+    uint32_t prefix = 0;
+    if (tile > 0) {
+      prefix = lookback(desc, tile, (uint32_t)tid, parity);
+      st_agent(my_desc, Desc<D>::make(kStInclusive, (D)(prefix + total), parity));
+    }
+    gbase[tid] = digit_base[tid] + prefix - excl;
+  }
+  __syncthreads();
+
+  // Tile-local scatter into LDS (sorted by digit, stable).
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    const uint32_t pos = off[w][dg[k]] + rk[k];
+    if ((full || wb + k * kWave < n) && pos < (uint32_t)TILE) {
+      u.keys[pos] = key[k];
+      if constexpr (HV) vstage[pos] = val[k];
+    }
+  }
+  __syncthreads();
+
+  // Coalesced write-out: consecutive threads, consecutive positions.
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = (uint32_t)(j * BLOCK + tid);
+    if (full || i < tile_cnt) {
+      const Bits kk = u.keys[i];
+      const uint32_t o = gbase[digit_of<KT>(kk, shift)] + i;
+      if (o < n) {
+        dst[o] = kk;
+        if constexpr (HV) vdst[o] = vstage[i];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sorts n <= BLOCK*IPT keys (and payloads) with one workgroup in LDS.
+// k_in may equal k_out.
+// ---------------------------------------------------------------------------
+template <int KT, typename V, int BLOCK, int IPT>
+__device__ __forceinline__ void small_sort_block(const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_out,
+                                                 const V* v_in, V* v_out, uint32_t n) {
+  using Bits = typename Codec<KT>::Bits;
+  constexpr int W = BLOCK / kWave;
+  constexpr int TILE = BLOCK * IPT;
+  constexpr bool HV = kHasVal<V>;
+  constexpr int P = (int)sizeof(Bits);
+  using VS = typename std::conditional<HV, V, uint32_t>::type;
+
+  __shared__ Bits keys[TILE];
+  __shared__ VS vals[HV ? TILE : 1];
+  __shared__ uint64_t peers[W][kRadix];
+  __shared__ uint32_t off[W][kRadix];
+  __shared__ uint32_t sums[4];
+  __shared__ int s_flag;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int w = tid / kWave;
+  for (int i = tid; i < W * kRadix; i += BLOCK) (&peers[0][0])[i] = 0;
+  for (uint32_t i = tid; i < n; i += BLOCK) {
+    keys[i] = k_in[i];
+    if constexpr (HV) vals[i] = v_in[i];
+  }
+  __syncthreads();
+
+  const uint32_t wb = (uint32_t)(w * IPT * kWave + lane);
+  for (int p = 0; p < P; ++p) {
+    const int shift = 8 * p;
+    for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
+    if (tid == 0) s_flag = 0;
+    Bits key[IPT];
+    VS val[IPT];
+    uint32_t dg[IPT], rk[IPT];
+    const uint32_t d_first = digit_of<KT>(keys[0], shift);
+    bool mixed = false;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      const uint32_t idx = wb + k * kWave;
+      key[k] = idx < n ? keys[idx] : Bits(0);
+      if constexpr (HV) val[k] = idx < n ? vals[idx] : VS(0);
+      dg[k] = digit_of<KT>(key[k], shift);
+      mixed |= (idx < n) && (dg[k] != d_first);
+    }
+    __syncthreads();
+    if (mixed) s_flag = 1;
+    __syncthreads();
+    if (s_flag == 0) continue;  // every key has the same byte: identity pass
+#pragma unroll
+    for (int k = 0; k < IPT; ++k)
+      if (wb + k * kWave < n) rk[k] = rank_wave(dg[k], peers[w], off[w]);
+    __syncthreads();
+    uint32_t total = 0;
+    if (tid < kRadix) {
+#pragma unroll
+      for (int i = 0; i < W; ++i) total += off[i][tid];
+    }
+    const uint32_t excl = block_excl_scan_256(total, sums, tid);
+    if (tid < kRadix) {
+      uint32_t run = excl;
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const uint32_t c = off[i][tid];
+        off[i][tid] = run;
+        run += c;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      const uint32_t pos = off[w][dg[k]] + rk[k];
+      if (wb + k * kWave < n && pos < (uint32_t)TILE) {
+        keys[pos] = key[k];
+        if constexpr (HV) vals[pos] = val[k];
+      }
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = tid; i < n; i += BLOCK) {
+    k_out[i] = keys[i];
+    if constexpr (HV) v_out[i] = vals[i];
+  }
+}
+
+}  // namespace mgs

@@ -1,0 +1,133 @@
+// LSD onesweep radix sort for gfx950 (MI355X).
+//
+// Replaces the thrust::sort / cub::DeviceRadixSort call sites of the
+// reference (src/tanasic_sort.cuh:264-266, src/gowanlock_sort.cuh:188-190,
+// :337-340, src/sort_benchmark.cu:89, src/local_gpu_sort_algorithm_benchmark.cu
+// :55-65).  Algorithm: Adinets & Merrill, "Onesweep" (2022) with the
+// decoupled look-back of Merrill & Garland (2016), re-designed for 64-lane
+// wavefronts:
+//
+//   init      zero the digit histograms and tile counters
+//   hist      one read of all keys -> the 256-bin histogram of every byte;
+//             also clears the look-back descriptors
+//   plan      exclusive scan of each histogram (global digit bases), marks
+//             bytes whose histogram has a single bin as trivial (identity
+//             permutation, skipped) and assigns ping-pong buffers so that the
+//             result lands in keys_out and the input is never written unless
+//             keys_in == keys_out
+//   pass p    (one launch per byte) each workgroup takes the next tile from an
+//             atomic counter, ranks its keys per wavefront with an LDS peer
+//             table (stable, no reliance on atomic ordering), publishes its
+//             digit counts, looks back over predecessor tiles for the global
+//             prefix, sorts the tile in LDS and writes each digit run with
+//             consecutive lanes on consecutive addresses
+//   fixup     copies the result into keys_out when the number of non-trivial
+//             passes left it in the scratch buffer
+//
+// Small inputs (one tile) are sorted by a single workgroup entirely in LDS.
+#pragma once
+
+#include "common.hpp"
+
+namespace mgs {
+
+constexpr int kMaxPasses = 8;
+
+// Buffer ids used by the pass plan.
+enum : int32_t { kBufIn = 0, kBufAlt = 1, kBufOut = 2 };
+
+struct PassPlan {
+  int32_t num_active;
+  int32_t k_of_pass[kMaxPasses];  // active index of pass p, -1 if trivial
+  int32_t src[kMaxPasses];        // indexed by active index
+  int32_t dst[kMaxPasses];
+  int32_t final_copy_src;         // -1: none
+  int32_t pad[3];
+};
+
+// Look-back descriptor: 2-bit status code in the top bits, count below.
+// The status code is rotated by the parity of the active pass index so that
+// the final state of one pass (all "inclusive") reads as "empty" in the next,
+// and the descriptor array is cleared only once per sort.
+template <typename D> struct Desc {
+  static constexpr int kShift = sizeof(D) * 8 - 2;
+  static constexpr D kMask = (D(1) << kShift) - 1;
+  __device__ static inline D make(uint32_t status, D value, int parity) {
+    D code = (D)((status + 2u * (uint32_t)parity) & 3u);
+    return (code << kShift) | (value & kMask);
+  }
+  __device__ static inline uint32_t status(D v, int parity) {
+    uint32_t code = (uint32_t)(v >> kShift);
+    return (code - 2u * (uint32_t)parity) & 3u;
+  }
+  __device__ static inline D value(D v) { return v & kMask; }
+};
+enum : uint32_t { kStEmpty = 0, kStAggregate = 1, kStInclusive = 2 };
+
+// ---------------------------------------------------------------------------
+// Tile geometry per (key bytes, value bytes).
+// ---------------------------------------------------------------------------
+template <int KB, int VB> struct TileCfg {
+  static constexpr int kBlock = 512;
+  static constexpr int kIpt = (KB + VB) <= 8 ? 16 : 8;
+  static constexpr int kTile = kBlock * kIpt;
+};
+
+// Single-workgroup LDS sort geometry.
+template <int KB, int VB> struct SmallCfg {
+  static constexpr int kBlock = 256;
+  static constexpr int kIpt = (KB + VB) <= 8 ? 16 : 8;
+  static constexpr int kTile = kBlock * kIpt;
+};
+
+// ---------------------------------------------------------------------------
+// Stable per-wavefront ranking of one item per lane.
+//
+// peers:  this wavefront's 256-entry table of 64-bit lane masks (all zero on
+//         entry and on exit)
+// cnt:    this wavefront's running count per digit
+// Returns the number of items with the same digit that precede this one in
+// the wavefront's order (earlier items, then lower lanes of this item).
+//
+// Lanes with equal digits find each other through an LDS atomic OR of their
+// lane bit into peers[d]; every lane then reads back the full mask, counts
+// the lower lanes in it (mbcnt) and the lowest lane advances cnt[d].  DS
+// instructions of one wavefront execute in order, so the read follows every
+// lane's OR, and the clear follows every lane's read.  A wave whose active
+// lanes all share one digit (runs in sorted input) skips the table.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint32_t* cnt) {
+  const uint64_t act = ballot64(true);
+  const uint32_t d0 = readfirstlane(d);
+  const uint64_t same = ballot64(d == d0);
+  if (same == act) {
+    const uint32_t below = mbcnt64(act);
+    const uint32_t base = ld_wg(&cnt[d0]);
+    if (below == 0) st_wg(&cnt[d0], base + (uint32_t)__popcll(act));
+    return base + below;
+  }
+  const int lane = lane_id();
+  or_wg(&peers[d], (uint64_t)1 << lane);
+  const uint64_t pm = ld_wg(&peers[d]);
+  st_wg(&peers[d], (uint64_t)0);
+  const uint32_t base = ld_wg(&cnt[d]);
+  const uint32_t below = mbcnt64(pm);
+  if (below == 0) st_wg(&cnt[d], base + (uint32_t)__popcll(pm));
+  return base + below;
+}
+
+// Exclusive scan of one value per digit held by threads 0..255 (4 waves).
+// `sums` is 4 words of LDS.  Must be called by every thread of the block
+// (contains barriers); threads >= 256 pass 0 and get garbage.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* sums, int tid) {
+  const uint32_t incl = wave_inclusive_sum(v);
+  if (tid < 256 && (tid & 63) == 63) sums[tid >> 6] = incl;
+  __syncthreads();
+  uint32_t base = 0;
+  const int w = tid >> 6;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) base += (i < w) ? sums[i] : 0u;
+  return base + incl - v;
+}
+
+}  // namespace mgs

@@ -1,0 +1,582 @@
+// Batched segmented sort: thousands of independent arrays in a fixed launch
+// sequence whose grid sizes depend only on (nseg, max_total_keys), so a call
+// can be captured once in a hipGraph and replayed for new data.
+//
+//   classify  one workgroup: bins each segment by length
+//               small  (len <= small_max)  -> one workgroup sorts it in LDS
+//               medium (len <= medium_max) -> one workgroup streams it through
+//                                             HBM pass by pass, no look-back
+//               large                      -> onesweep tiles chained by
+//                                             decoupled look-back inside the
+//                                             segment
+//             and lays the large segments' tiles out back to back
+//   clear     zeroes per-segment histograms and look-back descriptors
+//   small     persistent grid over the small bin
+//   medium    persistent grid over the medium bin
+//   hist      per-tile histograms of the large segments
+//   scan      per-segment digit bases
+//   pass p    every large-segment tile of byte p in one launch
+#include <type_traits>
+
+#include "internal.hpp"
+#include "radix_device.hpp"
+
+namespace mgs {
+
+namespace {
+
+struct SegCtl {
+  uint32_t n_small, n_medium, n_large, large_tiles;
+};
+
+constexpr int kClassifyBlock = 1024;
+
+// Exclusive scan of three counters across a 1024-thread block.
+__device__ inline void block_scan3(uint32_t a, uint32_t b, uint32_t c, uint32_t& ea, uint32_t& eb, uint32_t& ec,
+                                   uint32_t& ta, uint32_t& tb, uint32_t& tc, uint32_t (*sm)[16]) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t ia = wave_inclusive_sum(a), ib = wave_inclusive_sum(b), ic = wave_inclusive_sum(c);
+  if (lane == 63) {
+    sm[0][w] = ia;
+    sm[1][w] = ib;
+    sm[2][w] = ic;
+  }
+  __syncthreads();
+  uint32_t ba = 0, bb = 0, bc = 0, sa = 0, sb = 0, sc = 0;
+  for (int i = 0; i < kClassifyBlock / 64; ++i) {
+    if (i < w) {
+      ba += sm[0][i];
+      bb += sm[1][i];
+      bc += sm[2][i];
+    }
+    sa += sm[0][i];
+    sb += sm[1][i];
+    sc += sm[2][i];
+  }
+  ea = ba + ia - a;
+  eb = bb + ib - b;
+  ec = bc + ic - c;
+  ta = sa;
+  tb = sb;
+  tc = sc;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kClassifyBlock) void seg_classify_kernel(const uint64_t* __restrict__ offsets,
+                                                                      uint32_t nseg, uint32_t small_max,
+                                                                      uint32_t medium_max, uint32_t tile,
+                                                                      uint32_t* small_list, uint32_t* medium_list,
+                                                                      uint32_t* large_list, uint32_t* tile_start,
+                                                                      SegCtl* ctl, uint32_t* tile_counter) {
+  __shared__ uint32_t sm[3][16];
+  __shared__ uint32_t sm_t[16];
+  const int tid = threadIdx.x;
+  if (tid < kMaxPasses + 1) tile_counter[tid] = 0;
+  uint32_t run_s = 0, run_m = 0, run_l = 0, run_t = 0;
+  for (uint32_t base = 0; base < nseg; base += kClassifyBlock) {
+    const uint32_t s = base + tid;
+    uint32_t len = 0;
+    if (s < nseg) len = (uint32_t)(offsets[s + 1] - offsets[s]);
+    const uint32_t is_s = (s < nseg && len > 0 && len <= small_max) ? 1u : 0u;
+    const uint32_t is_m = (s < nseg && len > small_max && len <= medium_max) ? 1u : 0u;
+    const uint32_t is_l = (s < nseg && len > medium_max) ? 1u : 0u;
+    const uint32_t nt = is_l ? (len + tile - 1) / tile : 0u;
+    uint32_t es, em, el, ts, tm, tl;
+    block_scan3(is_s, is_m, is_l, es, em, el, ts, tm, tl, sm);
+    // tiles scan (separately, reusing the 3-way scan with zeros)
+    uint32_t et, d0, d1, tt, t0, t1;
+    block_scan3(nt, 0, 0, et, d0, d1, tt, t0, t1, sm);
+    if (is_s) small_list[run_s + es] = s;
+    if (is_m) medium_list[run_m + em] = s;
+    if (is_l) {
+      large_list[run_l + el] = s;
+      tile_start[run_l + el] = run_t + et;
+    }
+    run_s += ts;
+    run_m += tm;
+    run_l += tl;
+    run_t += tt;
+  }
+  if (tid == 0) {
+    ctl->n_small = run_s;
+    ctl->n_medium = run_m;
+    ctl->n_large = run_l;
+    ctl->large_tiles = run_t;
+    tile_start[run_l] = run_t;
+  }
+  (void)sm_t;
+}
+
+__global__ void seg_clear_kernel(uint32_t* __restrict__ hist, uint32_t hist_words_per_seg,
+                                 uint32_t* __restrict__ desc, uint32_t desc_words_per_tile,
+                                 const SegCtl* __restrict__ ctl) {
+  const size_t hw = (size_t)ctl->n_large * hist_words_per_seg;
+  const size_t dw = (size_t)ctl->large_tiles * desc_words_per_tile;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = g; i < hw; i += st) hist[i] = 0;
+  for (size_t i = g; i < dw; i += st) desc[i] = 0;
+}
+
+template <int KT, typename V, int BLOCK, int IPT>
+__global__ __launch_bounds__(BLOCK) void seg_small_kernel(const typename Codec<KT>::Bits* k_in,
+                                                          typename Codec<KT>::Bits* k_out, const V* v_in, V* v_out,
+                                                          const uint64_t* __restrict__ offsets,
+                                                          const uint32_t* __restrict__ list,
+                                                          const SegCtl* __restrict__ ctl) {
+  const uint32_t cnt = ctl->n_small;
+  for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+    const uint32_t s = list[i];
+    const uint64_t b = offsets[s];
+    const uint32_t len = (uint32_t)(offsets[s + 1] - b);
+    const V* vi = nullptr;
+    V* vo = nullptr;
+    if constexpr (kHasVal<V>) {
+      vi = v_in + b;
+      vo = v_out + b;
+    }
+    small_sort_block<KT, V, BLOCK, IPT>(k_in + b, k_out + b, vi, vo, len);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Medium bin: one workgroup sorts a whole segment, pass by pass, walking its
+// tiles in order and carrying the per-digit running offsets in LDS, so no
+// cross-workgroup look-back is needed.
+// ---------------------------------------------------------------------------
+template <int KT, typename V, int BLOCK, int IPT>
+__device__ void medium_sort_block(const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt,
+                                  typename Codec<KT>::Bits* k_out, const V* v_in, V* v_alt, V* v_out, uint32_t n) {
+  using Bits = typename Codec<KT>::Bits;
+  constexpr int W = BLOCK / kWave;
+  constexpr int TILE = BLOCK * IPT;
+  constexpr bool HV = kHasVal<V>;
+  constexpr int P = (int)sizeof(Bits);
+  using VS = typename std::conditional<HV, V, uint32_t>::type;
+  struct RankSmem {
+    uint64_t peers[W][kRadix];
+  };
+  union StageSmem {
+    RankSmem r;
+    Bits keys[TILE];
+  };
+  __shared__ StageSmem u;
+  __shared__ VS vstage[HV ? TILE : 1];
+  __shared__ uint32_t off[W][kRadix];
+  __shared__ uint32_t hist[P][kRadix];
+  __shared__ uint32_t run[kRadix];
+  __shared__ uint32_t gbase[kRadix];
+  __shared__ uint32_t sums[4];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int w = tid / kWave;
+
+  // Histogram of every byte.
+  for (int i = tid; i < P * kRadix; i += BLOCK) (&hist[0][0])[i] = 0;
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += BLOCK) {
+    const Bits k = k_in[i];
+#pragma unroll
+    for (int p = 0; p < P; ++p) atomicAdd(&hist[p][digit_of<KT>(k, 8 * p)], 1u);
+  }
+  __syncthreads();
+
+  for (int p = 0; p < P; ++p) {
+    const int shift = 8 * p;
+    const Bits* src = p == 0 ? k_in : ((p & 1) ? k_alt : k_out);
+    Bits* dst = (p & 1) ? k_out : k_alt;
+    const V* vsrc = nullptr;
+    V* vdst = nullptr;
+    if constexpr (HV) {
+      vsrc = p == 0 ? v_in : ((p & 1) ? v_alt : v_out);
+      vdst = (p & 1) ? v_out : v_alt;
+    }
+    {  // running digit offsets start at the exclusive histogram scan
+      const uint32_t c = tid < kRadix ? hist[p][tid] : 0u;
+      const uint32_t e = block_excl_scan_256(c, sums, tid);
+      if (tid < kRadix) run[tid] = e;
+    }
+    for (uint32_t tb = 0; tb < n; tb += TILE) {
+      for (int i = tid; i < W * kRadix; i += BLOCK) {
+        (&u.r.peers[0][0])[i] = 0;
+        (&off[0][0])[i] = 0;
+      }
+      __syncthreads();
+      const uint32_t tile_cnt = min((uint32_t)TILE, n - tb);
+      Bits key[IPT];
+      VS val[IPT];
+      uint32_t dg[IPT], rk[IPT];
+      const uint32_t wb = tb + (uint32_t)(w * IPT * kWave + lane);
+#pragma unroll
+      for (int k = 0; k < IPT; ++k) {
+        const uint32_t idx = wb + k * kWave;
+        key[k] = idx < n ? src[idx] : Bits(0);
+        if constexpr (HV) val[k] = idx < n ? vsrc[idx] : VS(0);
+        dg[k] = digit_of<KT>(key[k], shift);
+      }
+#pragma unroll
+      for (int k = 0; k < IPT; ++k)
+        if (wb + k * kWave < n) rk[k] = rank_wave(dg[k], u.r.peers[w], off[w]);
+      __syncthreads();
+      uint32_t total = 0;
+      if (tid < kRadix) {
+#pragma unroll
+        for (int i = 0; i < W; ++i) total += off[i][tid];
+      }
+      const uint32_t excl = block_excl_scan_256(total, sums, tid);
+      if (tid < kRadix) {
+        uint32_t r = excl;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+          const uint32_t c = off[i][tid];
+          off[i][tid] = r;
+          r += c;
+        }
+        gbase[tid] = run[tid] - excl;
+        run[tid] += total;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < IPT; ++k) {
+        const uint32_t pos = off[w][dg[k]] + rk[k];
+        if (wb + k * kWave < n && pos < (uint32_t)TILE) {
+          u.keys[pos] = key[k];
+          if constexpr (HV) vstage[pos] = val[k];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t i = (uint32_t)(j * BLOCK + tid);
+        if (i < tile_cnt) {
+          const Bits kk = u.keys[i];
+          const uint32_t o = gbase[digit_of<KT>(kk, shift)] + i;
+          if (o < n) {
+            dst[o] = kk;
+            if constexpr (HV) vdst[o] = vstage[i];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+template <int KT, typename V, int BLOCK, int IPT>
+__global__ __launch_bounds__(BLOCK) void seg_medium_kernel(const typename Codec<KT>::Bits* k_in,
+                                                           typename Codec<KT>::Bits* k_alt,
+                                                           typename Codec<KT>::Bits* k_out, const V* v_in, V* v_alt,
+                                                           V* v_out, const uint64_t* __restrict__ offsets,
+                                                           const uint32_t* __restrict__ list,
+                                                           const SegCtl* __restrict__ ctl) {
+  const uint32_t cnt = ctl->n_medium;
+  const uint64_t base0 = offsets[0];
+  for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+    const uint32_t s = list[i];
+    const uint64_t b = offsets[s];
+    const uint32_t len = (uint32_t)(offsets[s + 1] - b);
+    const uint64_t ab = b - base0;  // scratch is indexed from offsets[0]
+    const V* vi = nullptr;
+    V *va = nullptr, *vo = nullptr;
+    if constexpr (kHasVal<V>) {
+      vi = v_in + b;
+      va = v_alt + ab;
+      vo = v_out + b;
+    }
+    medium_sort_block<KT, V, BLOCK, IPT>(k_in + b, k_alt + ab, k_out + b, vi, va, vo, len);
+    __syncthreads();
+  }
+}
+
+// Large bin: locate tile g among the large segments.
+__device__ __forceinline__ uint32_t find_large(const uint32_t* tile_start, uint32_t n_large, uint32_t g) {
+  uint32_t lo = 0, hi = n_large;  // largest li with tile_start[li] <= g
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (tile_start[mid] <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int KT, int BLOCK, int TILE>
+__global__ __launch_bounds__(BLOCK) void seg_hist_kernel(const typename Codec<KT>::Bits* __restrict__ k_in,
+                                                         const uint64_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ large_list,
+                                                         const uint32_t* __restrict__ tile_start,
+                                                         uint32_t* __restrict__ seg_hist,
+                                                         const SegCtl* __restrict__ ctl) {
+  using Bits = typename Codec<KT>::Bits;
+  constexpr int P = (int)sizeof(Bits);
+  __shared__ uint32_t h[P][kRadix];
+  const uint32_t g = blockIdx.x;
+  if (g >= ctl->large_tiles) return;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < P * kRadix; i += BLOCK) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t li = find_large(tile_start, ctl->n_large, g);
+  const uint32_t s = large_list[li];
+  const uint64_t b = offsets[s];
+  const uint32_t len = (uint32_t)(offsets[s + 1] - b);
+  const uint32_t t0 = (g - tile_start[li]) * (uint32_t)TILE;
+  const uint32_t t1 = min(len, t0 + (uint32_t)TILE);
+  for (uint32_t i = t0 + tid; i < t1; i += BLOCK) {
+    const Bits k = k_in[b + i];
+#pragma unroll
+    for (int p = 0; p < P; ++p) atomicAdd(&h[p][digit_of<KT>(k, 8 * p)], 1u);
+  }
+  __syncthreads();
+  uint32_t* dsth = seg_hist + (size_t)li * P * kRadix;
+  for (int i = tid; i < P * kRadix; i += BLOCK) {
+    const uint32_t c = (&h[0][0])[i];
+    if (c) atomicAdd(&dsth[i], c);
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void seg_scan_kernel(const uint32_t* seg_hist, uint32_t* seg_base,
+                                                       const SegCtl* __restrict__ ctl) {
+  __shared__ uint32_t sums[4];
+  const uint32_t li = blockIdx.x;
+  if (li >= ctl->n_large) return;
+  const int tid = threadIdx.x;
+  for (int p = 0; p < P; ++p) {
+    const size_t o = ((size_t)li * P + p) * kRadix + tid;
+    const uint32_t c = seg_hist[o];
+    const uint32_t e = block_excl_scan_256(c, sums, tid);
+    __syncthreads();
+    seg_base[o] = e;  // may alias seg_hist: every read of this row is done
+  }
+}
+
+template <int KT, typename V, int BLOCK, int IPT, typename D>
+__global__ __launch_bounds__(BLOCK) void seg_pass_kernel(const typename Codec<KT>::Bits* k_in,
+                                                         typename Codec<KT>::Bits* k_alt,
+                                                         typename Codec<KT>::Bits* k_out, const V* v_in, V* v_alt,
+                                                         V* v_out, const uint64_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ large_list,
+                                                         const uint32_t* __restrict__ tile_start,
+                                                         const uint32_t* __restrict__ seg_base,
+                                                         uint32_t* __restrict__ tile_counter, D* __restrict__ desc,
+                                                         const SegCtl* __restrict__ ctl, int pass) {
+  using Bits = typename Codec<KT>::Bits;
+  constexpr int P = (int)sizeof(Bits);
+  __shared__ uint32_t s_tile;
+  if (threadIdx.x == 0) s_tile = atomicAdd(&tile_counter[pass], 1u);
+  __syncthreads();
+  const uint32_t g = s_tile;
+  if (g >= ctl->large_tiles) return;
+  const uint32_t li = find_large(tile_start, ctl->n_large, g);
+  const uint32_t s = large_list[li];
+  const uint64_t b = offsets[s];
+  const uint64_t ab = b - offsets[0];
+  const uint32_t len = (uint32_t)(offsets[s + 1] - b);
+  const Bits* src;
+  Bits* dst;
+  if (pass == 0) {
+    src = k_in + b;
+    dst = k_alt + ab;
+  } else if (pass & 1) {
+    src = k_alt + ab;
+    dst = k_out + b;
+  } else {
+    src = k_out + b;
+    dst = k_alt + ab;
+  }
+  const V* vsrc = nullptr;
+  V* vdst = nullptr;
+  if constexpr (kHasVal<V>) {
+    if (pass == 0) {
+      vsrc = v_in + b;
+      vdst = v_alt + ab;
+    } else if (pass & 1) {
+      vsrc = v_alt + ab;
+      vdst = v_out + b;
+    } else {
+      vsrc = v_out + b;
+      vdst = v_alt + ab;
+    }
+  }
+  const uint32_t t0 = tile_start[li];
+  onesweep_tile<KT, V, BLOCK, IPT, D>(src, dst, vsrc, vdst, len, g - t0, desc + (size_t)t0 * kRadix,
+                                      seg_base + ((size_t)li * P + pass) * kRadix, pass * 8, pass & 1);
+}
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+struct SegLayout {
+  size_t alt_k, alt_v, ctl, small_list, medium_list, large_list, tile_start, seg_hist, seg_base, tile_counter, desc,
+      total;
+  uint32_t max_large, tiles_ub;
+  bool wide_desc;
+};
+
+template <int KB, int VB>
+struct SegCfg {
+  using T = TileCfg<KB, VB>;
+  using S = SmallCfg<KB, VB>;
+  static constexpr int kSmallMax = S::kTile;
+  static constexpr int kMediumMax = 1 << 17;
+};
+
+// Large segments always hold more than kMinLargeExclusive keys, whatever the
+// tuning, so the scratch size depends only on (max_total, nseg).
+constexpr uint32_t kMinLargeExclusive = 256;
+
+template <int KB, int VB>
+SegLayout seg_layout(size_t max_total, size_t nseg) {
+  using C = SegCfg<KB, VB>;
+  constexpr int P = KB;
+  SegLayout L{};
+  L.max_large = (uint32_t)std::min<size_t>(nseg, max_total / (kMinLargeExclusive + 1) + 1);
+  L.tiles_ub = (uint32_t)(max_total / C::T::kTile + L.max_large + 1);
+  L.wide_desc = max_total > (size_t(1) << 30);
+  size_t o = 0;
+  L.alt_k = o;
+  o = align_up(o + max_total * KB);
+  L.alt_v = o;
+  o = align_up(o + max_total * VB);
+  L.ctl = o;
+  o = align_up(o + sizeof(SegCtl));
+  L.small_list = o;
+  o = align_up(o + nseg * 4);
+  L.medium_list = o;
+  o = align_up(o + nseg * 4);
+  L.large_list = o;
+  o = align_up(o + nseg * 4);
+  L.tile_start = o;
+  o = align_up(o + (nseg + 1) * 4);
+  L.seg_hist = o;  // histograms, scanned in place into digit bases
+  o = align_up(o + (size_t)L.max_large * P * kRadix * 4);
+  L.seg_base = L.seg_hist;
+  L.tile_counter = o;
+  o = align_up(o + (kMaxPasses + 1) * 4);
+  L.desc = o;
+  o = align_up(o + (size_t)L.tiles_ub * kRadix * (L.wide_desc ? 8 : 4));
+  L.total = o;
+  return L;
+}
+
+template <int KT, int VT>
+int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const uint64_t* offsets, size_t nseg,
+              size_t max_total, const mgs_tuning* tuning, void* temp, size_t temp_bytes, hipStream_t s) {
+  using Bits = typename Codec<KT>::Bits;
+  using V = typename ValOf<VT>::T;
+  constexpr int KB = sizeof(Bits);
+  constexpr int VB = kValBytes<V>;
+  constexpr int P = KB;
+  using C = SegCfg<KB, VB>;
+  using TC = typename C::T;
+  using SC = typename C::S;
+  using VP = typename std::conditional<kHasVal<V>, V, uint32_t>::type;
+
+  uint32_t small_max = C::kSmallMax, medium_max = C::kMediumMax;
+  if (tuning) {
+    if (tuning->small_max > 0) small_max = std::min<uint32_t>((uint32_t)tuning->small_max, C::kSmallMax);
+    if (tuning->medium_max > 0) medium_max = (uint32_t)tuning->medium_max;
+  }
+  medium_max = std::max({medium_max, small_max, kMinLargeExclusive});
+  const SegLayout L = seg_layout<KB, VB>(max_total, nseg);
+  if (temp == nullptr || temp_bytes < L.total) return MGS_ERR_TEMP_TOO_SMALL;
+  char* t = static_cast<char*>(temp);
+  Bits* ka = reinterpret_cast<Bits*>(t + L.alt_k);
+  VP* va = reinterpret_cast<VP*>(t + L.alt_v);
+  SegCtl* ctl = reinterpret_cast<SegCtl*>(t + L.ctl);
+  uint32_t* small_list = reinterpret_cast<uint32_t*>(t + L.small_list);
+  uint32_t* medium_list = reinterpret_cast<uint32_t*>(t + L.medium_list);
+  uint32_t* large_list = reinterpret_cast<uint32_t*>(t + L.large_list);
+  uint32_t* tile_start = reinterpret_cast<uint32_t*>(t + L.tile_start);
+  uint32_t* seg_hist = reinterpret_cast<uint32_t*>(t + L.seg_hist);
+  uint32_t* seg_base = reinterpret_cast<uint32_t*>(t + L.seg_base);
+  uint32_t* tcount = reinterpret_cast<uint32_t*>(t + L.tile_counter);
+  void* desc = t + L.desc;
+
+  const Bits* ki = static_cast<const Bits*>(kin);
+  Bits* ko = static_cast<Bits*>(kout);
+  const V* vi = static_cast<const V*>(vin);
+  V* vo = static_cast<V*>(vout);
+
+  seg_classify_kernel<<<1, kClassifyBlock, 0, s>>>(offsets, (uint32_t)nseg, small_max, medium_max, TC::kTile,
+                                                   small_list, medium_list, large_list, tile_start, ctl, tcount);
+  seg_clear_kernel<<<2048, 256, 0, s>>>(seg_hist, P * kRadix, (uint32_t*)desc, kRadix * (L.wide_desc ? 2 : 1), ctl);
+  {
+    const unsigned grid = (unsigned)std::min<size_t>(nseg, 8192);
+    seg_small_kernel<KT, V, SC::kBlock, SC::kIpt><<<grid, SC::kBlock, 0, s>>>(ki, ko, vi, vo, offsets, small_list, ctl);
+    seg_medium_kernel<KT, V, TC::kBlock, TC::kIpt>
+        <<<grid, TC::kBlock, 0, s>>>(ki, ka, ko, vi, (V*)va, vo, offsets, medium_list, ctl);
+  }
+  seg_hist_kernel<KT, 256, TC::kTile><<<L.tiles_ub, 256, 0, s>>>(ki, offsets, large_list, tile_start, seg_hist, ctl);
+  seg_scan_kernel<P><<<L.max_large, 256, 0, s>>>(seg_hist, seg_base, ctl);
+  for (int p = 0; p < P; ++p) {
+    if (L.wide_desc)
+      seg_pass_kernel<KT, V, TC::kBlock, TC::kIpt, uint64_t><<<L.tiles_ub, TC::kBlock, 0, s>>>(
+          ki, ka, ko, vi, (V*)va, vo, offsets, large_list, tile_start, seg_base, tcount, (uint64_t*)desc, ctl, p);
+    else
+      seg_pass_kernel<KT, V, TC::kBlock, TC::kIpt, uint32_t><<<L.tiles_ub, TC::kBlock, 0, s>>>(
+          ki, ka, ko, vi, (V*)va, vo, offsets, large_list, tile_start, seg_base, tcount, (uint32_t*)desc, ctl, p);
+  }
+  return check_launch();
+}
+
+template <int KT>
+int seg_by_val(mgs_val vt, const void* kin, void* kout, const void* vin, void* vout, const uint64_t* offsets,
+               size_t nseg, size_t max_total, const mgs_tuning* tuning, void* temp, size_t temp_bytes, hipStream_t s) {
+  switch (vt) {
+    case MGS_VAL_NONE:
+      return seg_typed<KT, MGS_VAL_NONE>(kin, kout, vin, vout, offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+    case MGS_VAL_U32:
+      return seg_typed<KT, MGS_VAL_U32>(kin, kout, vin, vout, offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+    case MGS_VAL_U64:
+      return seg_typed<KT, MGS_VAL_U64>(kin, kout, vin, vout, offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+  }
+  return MGS_ERR_BAD_TYPE;
+}
+
+template <int KB, int VB>
+size_t seg_bytes(size_t max_total, size_t nseg) {
+  return seg_layout<KB, VB>(max_total, nseg).total;
+}
+
+}  // namespace
+
+size_t segmented_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total, size_t nseg) {
+  const int kb = key_bytes(kt), vb = val_bytes(vt);
+  if (kb < 0 || vb < 0) return 0;
+  if (kb == 4) {
+    if (vb == 0) return seg_bytes<4, 0>(max_total, nseg);
+    if (vb == 4) return seg_bytes<4, 4>(max_total, nseg);
+    return seg_bytes<4, 8>(max_total, nseg);
+  }
+  if (vb == 0) return seg_bytes<8, 0>(max_total, nseg);
+  if (vb == 4) return seg_bytes<8, 4>(max_total, nseg);
+  return seg_bytes<8, 8>(max_total, nseg);
+}
+
+int segmented_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout,
+                       const uint64_t* d_offsets, size_t nseg, size_t max_total, const mgs_tuning* tuning,
+                       void* temp, size_t temp_bytes, hipStream_t s) {
+  switch (kt) {
+    case MGS_KEY_U32:
+      return seg_by_val<MGS_KEY_U32>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+    case MGS_KEY_U64:
+      return seg_by_val<MGS_KEY_U64>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+    case MGS_KEY_I32:
+      return seg_by_val<MGS_KEY_I32>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+    case MGS_KEY_I64:
+      return seg_by_val<MGS_KEY_I64>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+    case MGS_KEY_F32:
+      return seg_by_val<MGS_KEY_F32>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+    case MGS_KEY_F64:
+      return seg_by_val<MGS_KEY_F64>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+  }
+  return MGS_ERR_BAD_TYPE;
+}
+
+}  // namespace mgs
