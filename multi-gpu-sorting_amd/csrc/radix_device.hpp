@@ -10,28 +10,50 @@ namespace mgs {
 
 constexpr uint32_t kLookbackSpinLimit = 1u << 24;
 
+// Wavefront ranking used by every kernel: one LDS atomic per item by
+// default; -DMGS_RANK_PEERS selects the peer-mask table instead.
+#ifdef MGS_RANK_PEERS
+#define MGS_RANK(d, peers, cnt) rank_wave((d), (peers), (cnt))
+#else
+#define MGS_RANK(d, peers, cnt) rank_wave_atomic((d), (cnt))
+#endif
+
 // Look-back over predecessor tiles of one chain for one digit: sums
 // aggregates until an inclusive prefix is found.  `desc` points at the
 // chain's first tile.
 template <typename D>
 __device__ __forceinline__ uint32_t lookback(const D* desc, uint32_t tile, uint32_t d, int parity) {
+  // Reads kWindow predecessors per round trip (independent loads in flight),
+  // then consumes them in order.
+  constexpr int kWindow = 4;
   uint32_t prefix = 0;
-  uint32_t t = tile - 1;
+  int64_t t = (int64_t)tile - 1;
   uint32_t spins = 0;
   for (;;) {
-    const D v = ld_agent(desc + (size_t)t * kRadix + d);
-    const uint32_t st = Desc<D>::status(v, parity);
-    if (st == kStEmpty) {
+    D v[kWindow];
+#pragma unroll
+    for (int j = 0; j < kWindow; ++j)
+      v[j] = (t - j >= 0) ? ld_agent(desc + (size_t)(t - j) * kRadix + d) : D(0);
+    int consumed = 0;
+    bool done = false;
+#pragma unroll
+    for (int j = 0; j < kWindow; ++j) {
+      if (done || consumed != j || t - j < 0) continue;
+      const uint32_t st = Desc<D>::status(v[j], parity);
+      if (st == kStEmpty) continue;
+      prefix += (uint32_t)Desc<D>::value(v[j]);
+      consumed = j + 1;
+      if (st == kStInclusive) done = true;
+    }
+    if (done) break;
+    if (consumed == 0) {
       // Bounded wait: a predecessor tile always publishes within microseconds;
       // the bound only keeps a corrupted descriptor array from hanging the GPU
       // (the result is then wrong, every store stays bounds-checked).
-      if (++spins > kLookbackSpinLimit) break;
+      if (++spins > kLookbackSpinLimit || t < 0) break;
       __builtin_amdgcn_s_sleep(1);
-      continue;
     }
-    prefix += (uint32_t)Desc<D>::value(v);
-    if (st == kStInclusive) break;
-    --t;
+    t -= consumed;
   }
   return prefix;
 }
@@ -39,17 +61,24 @@ __device__ __forceinline__ uint32_t lookback(const D* desc, uint32_t tile, uint3
 // ---------------------------------------------------------------------------
 // One onesweep tile.
 //
-// src/dst/vsrc/vdst  the chain's arrays (index 0 = first key of the chain)
-// n                  keys in the chain
-// tile               tile index inside the chain
-// desc               the chain's descriptors (tile-major, 256 per tile)
-// digit_base         256 exclusive digit bases of the chain for this byte
+// A chain is a run of consecutive tiles whose digit counts are prefix-summed
+// by look-back; the chain's starting offset of every digit is known before
+// the pass (digit_base), so different chains never wait on each other.
+//
+// src/vsrc    the chain's input (index 0 = first key of the chain)
+// n           keys in the chain
+// dst/vdst    output arrays; digit_base indexes them directly
+// n_dst       length of dst (bounds check of every store)
+// tile        tile index inside the chain
+// desc        the chain's descriptors (tile-major, 256 per tile)
+// digit_base  256 output offsets of the chain's first key of each digit
 // ---------------------------------------------------------------------------
 template <int KT, typename V, int BLOCK, int IPT, typename D>
 __device__ __forceinline__ void onesweep_tile(const typename Codec<KT>::Bits* __restrict__ src,
                                               typename Codec<KT>::Bits* __restrict__ dst, const V* __restrict__ vsrc,
-                                              V* __restrict__ vdst, uint32_t n, uint32_t tile, D* __restrict__ desc,
-                                              const uint32_t* __restrict__ digit_base, int shift, int parity) {
+                                              V* __restrict__ vdst, uint32_t n, uint32_t n_dst, uint32_t tile,
+                                              D* __restrict__ desc, const uint32_t* __restrict__ digit_base, int shift,
+                                              int parity) {
   using Bits = typename Codec<KT>::Bits;
   constexpr int W = BLOCK / kWave;
   constexpr int TILE = BLOCK * IPT;
@@ -106,13 +135,14 @@ __device__ __forceinline__ void onesweep_tile(const typename Codec<KT>::Bits* __
 
   uint64_t* pt = u.r.peers[w];
   uint32_t* ct = off[w];
+  (void)pt;
   if (full) {
 #pragma unroll
-    for (int k = 0; k < IPT; ++k) rk[k] = rank_wave(dg[k], pt, ct);
+    for (int k = 0; k < IPT; ++k) rk[k] = MGS_RANK(dg[k], pt, ct);
   } else {
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
-      if (wb + k * kWave < n) rk[k] = rank_wave(dg[k], pt, ct);
+      if (wb + k * kWave < n) rk[k] = MGS_RANK(dg[k], pt, ct);
     }
   }
   __syncthreads();
@@ -161,7 +191,7 @@ __device__ __forceinline__ void onesweep_tile(const typename Codec<KT>::Bits* __
     if (full || i < tile_cnt) {
       const Bits kk = u.keys[i];
       const uint32_t o = gbase[digit_of<KT>(kk, shift)] + i;
-      if (o < n) {
+      if (o < n_dst) {
         dst[o] = kk;
         if constexpr (HV) vdst[o] = vstage[i];
       }
@@ -224,7 +254,7 @@ __device__ __forceinline__ void small_sort_block(const typename Codec<KT>::Bits*
     if (s_flag == 0) continue;  // every key has the same byte: identity pass
 #pragma unroll
     for (int k = 0; k < IPT; ++k)
-      if (wb + k * kWave < n) rk[k] = rank_wave(dg[k], peers[w], off[w]);
+      if (wb + k * kWave < n) rk[k] = MGS_RANK(dg[k], peers[w], off[w]);
     __syncthreads();
     uint32_t total = 0;
     if (tid < kRadix) {

@@ -218,7 +218,7 @@ __device__ void medium_sort_block(const typename Codec<KT>::Bits* k_in, typename
       }
 #pragma unroll
       for (int k = 0; k < IPT; ++k)
-        if (wb + k * kWave < n) rk[k] = rank_wave(dg[k], u.r.peers[w], off[w]);
+        if (wb + k * kWave < n) rk[k] = MGS_RANK(dg[k], u.r.peers[w], off[w]);
       __syncthreads();
       uint32_t total = 0;
       if (tid < kRadix) {
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(BLOCK) void seg_pass_kernel(const typename Codec<KT
     }
   }
   const uint32_t t0 = tile_start[li];
-  onesweep_tile<KT, V, BLOCK, IPT, D>(src, dst, vsrc, vdst, len, g - t0, desc + (size_t)t0 * kRadix,
+  onesweep_tile<KT, V, BLOCK, IPT, D>(src, dst, vsrc, vdst, len, len, g - t0, desc + (size_t)t0 * kRadix,
                                       seg_base + ((size_t)li * P + pass) * kRadix, pass * 8, pass & 1);
 }
 
