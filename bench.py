@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Headline benchmark: sort 2^30 uniform u32 keys in HBM on one MI355X.
+
+One step = one call of mgs_sort (libmgsort, HIP kernels) over 2^30 keys
+already resident in HBM, written out of place so every step sorts the same
+unsorted input.  Keys come from the reference's DataGenerator (uniform,
+seed 2147483647; libmgsort_host).  Prints one JSON line:
+
+  value          whole-job GKeys/s (keys / wall time of the K timed steps)
+  roofline       the dominant kernel (the LSD pass kernel): algorithmic bytes
+                 per launch (8 B per key: one read and one write of the keys)
+                 over its average launch time, from HIP events recorded around
+                 each launch on the library's stream during the timed steps;
+                 `traffic` is the HBM bytes per launch measured by rocprofv3
+                 (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a plain copy),
+                 read from profiles/ when present
+  cpu_baseline   the oracle's sort (oracle/oracle_sort.c, one core) on a
+                 bounded sample of the same workload
+  cpu_parallel   PARADIS and GNU parallel sort on all host cores (sample)
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
+"""
+import argparse
+import ctypes
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "multi-gpu-sorting_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+SEED = 2147483647
+KEY_BYTES = 4
+ALG_BYTES_PER_KEY = KEY_BYTES + KEY_BYTES * 2 * KEY_BYTES  # 36: histogram read + 4 x (read + write)
+PASS_BYTES_PER_KEY = 2 * KEY_BYTES  # one LSD pass: read + write
+PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_lib():
+    path = os.path.join(ROOT, "multi-gpu-sorting_amd", "lib", "libmgsort_host.so")
+    L = ctypes.CDLL(path)
+    c, vp, sz = ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t
+    L.mgh_generate.argtypes = [c, c, ctypes.c_uint32, vp, sz]
+    L.mgh_generate.restype = ctypes.c_int
+    L.mgh_paradis_sort.argtypes = [c, vp, sz, ctypes.c_int]
+    L.mgh_gnu_parallel_sort.argtypes = [c, vp, sz, ctypes.c_int]
+    L.mgh_event_create.restype = vp
+    L.mgh_event_destroy.argtypes = [vp]
+    L.mgh_event_elapsed_ms.argtypes = [vp, vp]
+    L.mgh_event_elapsed_ms.restype = ctypes.c_float
+    return L
+
+
+def generate(H, n, np):
+    keys = np.empty(n, dtype=np.uint32)
+    rc = H.mgh_generate(b"u32", b"uniform", SEED, keys.ctypes.data, n)
+    if rc != 0:
+        raise RuntimeError(f"mgh_generate failed ({rc})")
+    return keys
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the pass kernel, if a calibrated rocprofv3
+    summary is committed (profiles/*pass_kernel*2e30*.txt: 'traffic_bytes: X')."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for f in sorted(os.listdir(d)):
+        if "pass_kernel" in f and f.endswith(".txt"):
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(os.path.join(d, f)).read())
+            if m:
+                return float(m.group(1))
+    return None
+
+
+def cpu_baseline(H, np):
+    """Oracle sort (one core) and the parallel CPU sorts on bounded samples."""
+    import pyoracle
+    out = {}
+    n = 1 << 25
+    keys = generate(H, n, np)
+    reps, t_total = 0, 0.0
+    while reps < 8 and t_total < 12.0:
+        t0 = time.perf_counter()
+        k, _ = pyoracle.sort(pyoracle.K_U32, keys)
+        t_total += time.perf_counter() - t0
+        reps += 1
+    assert np.all(k[1:] >= k[:-1])
+    out["cpu_baseline"] = {"value": round(n * reps / t_total / 1e9, 4), "unit": "GKeys/s", "cores": 1,
+                           "kind": "port",
+                           "sample": f"{reps} sorts of 2^25 uniform u32 keys (reference DataGenerator, seed {SEED}) "
+                                     f"by the oracle's byte-wise LSD counting sort, {t_total:.1f} s"}
+    threads = len(os.sched_getaffinity(0))
+    n2 = 1 << 28
+    keys = generate(H, n2, np)
+    par = {}
+    for name, fn in (("paradis", H.mgh_paradis_sort), ("gnu_parallel_sort", H.mgh_gnu_parallel_sort)):
+        k = keys.copy()
+        t0 = time.perf_counter()
+        fn(b"u32", k.ctypes.data, n2, threads)
+        dt = time.perf_counter() - t0
+        assert np.all(k[1:] >= k[:-1])
+        par[name] = round(n2 / dt / 1e9, 4)
+    out["cpu_parallel"] = {**par, "unit": "GKeys/s", "threads": threads,
+                           "sample": "2^28 uniform u32 keys, one sort each"}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 30)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: the sort path of this build runs on one GPU "
+              "(multi-GPU exchange is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import mgsort
+
+    H = host_lib()
+    n = args.n
+    log(f"generating {n} keys")
+    keys = generate(H, n, np)
+    src = torch.from_numpy(keys.view(np.int32)).cuda()
+    out = torch.empty_like(src)
+    temp = mgsort.temp_buffer(mgsort.lib.mgs_sort_temp_bytes(mgsort.KEY_U32, mgsort.VAL_NONE, n), "cuda")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    L = mgsort.lib
+    marks = ctypes.c_int(0)
+    rc = L.mgs_sort_timed(0, 0, src.data_ptr(), out.data_ptr(), None, None, n, temp.data_ptr(), temp.numel(), sp,
+                          None, 0, ctypes.byref(marks))
+    if rc != 0:
+        raise RuntimeError(f"mgs_sort_timed: {L.mgs_status_str(rc)}")
+    nm = marks.value
+    # stage order of the launch sequence: init, upfront, 4 passes, fixup
+    stage_names = ["init", "upfront"] + [f"pass{p}" for p in range(KEY_BYTES)] + ["fixup"]
+    events = [[H.mgh_event_create() for _ in range(nm)] for _ in range(args.steps)]
+    ev_arrays = [(ctypes.c_void_p * nm)(*e) for e in events]
+
+    for _ in range(args.warmup):
+        mgsort.sort("u32", src, keys_out=out, temp=temp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        rc = L.mgs_sort_timed(0, 0, src.data_ptr(), out.data_ptr(), None, None, n, temp.data_ptr(), temp.numel(), sp,
+                              ev_arrays[k], nm, ctypes.byref(marks))
+        if rc != 0:
+            raise RuntimeError(f"mgs_sort_timed: {L.mgs_status_str(rc)}")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # per-stage averages from the events
+    stage_ms = {}
+    for i in range(nm - 1):
+        vals = [H.mgh_event_elapsed_ms(events[k][i], events[k][i + 1]) for k in range(args.steps)]
+        name = stage_names[i] if i < len(stage_names) else f"stage{i}"
+        stage_ms[name] = sum(vals) / len(vals)
+    for e in events:
+        for x in e:
+            H.mgh_event_destroy(x)
+
+    # correctness of the last step's output: sorted, and the same multiset
+    o64 = out.to(torch.int64) & 0xFFFFFFFF
+    s64 = src.to(torch.int64) & 0xFFFFFFFF
+    ok = bool((o64[1:] >= o64[:-1]).all().item()) and int(o64.sum().item()) == int(s64.sum().item()) and \
+        int((o64 * o64 % 1000003).sum().item()) == int((s64 * s64 % 1000003).sum().item())
+    del o64, s64
+    if not ok:
+        raise RuntimeError("sorted output failed the check")
+
+    pass_ms = [stage_ms[f"pass{p}"] for p in range(KEY_BYTES)]
+    avg_pass_ms = sum(pass_ms) / len(pass_ms)
+    achieved = n * PASS_BYTES_PER_KEY / (avg_pass_ms / 1e3) / 1e9
+    value = n / (ms_per_step / 1e3) / 1e9
+    result = {
+        "metric": "GKeys/s & HBM GB/s sorting 2^30 u32 on 1 MI355X; batched p99 latency vs plain",
+        "value": round(value, 3),
+        "unit": "GKeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic: reference DataGenerator uniform u32, seed 2147483647, resident in HBM",
+        "config": {"workload": f"in-HBM sort of {n} uniform u32 keys (out of place, stable LSD radix)",
+                   "n": n, "key": "u32", "values": "none", "distribution": "uniform", "seed": SEED},
+        "hbm_gbps_alg": round(n * ALG_BYTES_PER_KEY / (ms_per_step / 1e3) / 1e9, 1),
+        "roofline_frac_whole_sort": round(n * ALG_BYTES_PER_KEY / (ms_per_step / 1e3) / 1e9 / PEAK_GBPS, 4),
+        "roofline": {"bound": "hbm", "kernel": "pass_kernel (one LSD byte pass)",
+                     "achieved": round(achieved, 1), "peak": PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_GBPS, 4), "traffic": traffic_from_profiles(),
+                     "alg_bytes_per_launch": n * PASS_BYTES_PER_KEY, "avg_launch_ms": round(avg_pass_ms, 4)},
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+    }
+    if not args.no_cpu:
+        del src, out, temp
+        torch.cuda.empty_cache()
+        result.update(cpu_baseline(H, np))
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -91,6 +91,18 @@ int mgs_sort(mgs_key kt, mgs_val vt,
              const void* vals_in, void* vals_out,
              size_t n, void* temp, size_t temp_bytes, void* stream);
 
+/* Measurement variant of mgs_sort (no reference counterpart; used by the
+ * benchmark to time each kernel on the stream it runs on): enqueues the same
+ * kernels and records events[i] (hipEvent_t handles created by the caller)
+ * before the i-th launch and one after the last, up to num_events.
+ * *num_marks receives the number of marks the sequence has (stages + 1), so
+ * a first call with num_events = 0 sizes the array. */
+int mgs_sort_timed(mgs_key kt, mgs_val vt,
+                   const void* keys_in, void* keys_out,
+                   const void* vals_in, void* vals_out,
+                   size_t n, void* temp, size_t temp_bytes, void* stream,
+                   void* const* events, int num_events, int* num_marks);
+
 /* --------------------------------------------------------------------------
  * Merge-path merge.  Replaces thrust::merge at reference
  * src/tanasic_sort.cuh:163-167 and src/local_gpu_merge_algorithm_benchmark.cu

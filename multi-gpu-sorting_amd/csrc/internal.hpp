@@ -37,7 +37,8 @@ inline int val_bytes(mgs_val vt) {
 
 size_t sort_temp_bytes(mgs_key kt, mgs_val vt, size_t n);
 int sort_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout, size_t n,
-                  void* temp, size_t temp_bytes, hipStream_t s);
+                  void* temp, size_t temp_bytes, hipStream_t s, hipEvent_t* events = nullptr, int num_events = 0,
+                  int* num_marks = nullptr);
 
 size_t merge_temp_bytes(mgs_key kt, mgs_val vt, size_t na, size_t nb);
 int merge_dispatch(mgs_key kt, mgs_val vt, const void* ak, const void* av, size_t na, const void* bk, const void* bv,

@@ -11,189 +11,122 @@ namespace mgs {
 // ---------------------------------------------------------------------------
 // Pipeline (one call, fixed launch sequence, no host synchronisation):
 //
-//   init       zero histograms and tile counters
-//   hist       one read: per-stripe 256-bin histograms of every byte
-//              (stripes = S contiguous runs of tiles of the input)
-//   plan       per byte: stripe output offsets (in place), global totals,
-//              "trivial" flag (one digit holds every key)
-//   msd        picks the MSD byte m = highest non-trivial byte; lays out the
-//              256 buckets of byte m and their tiles
-//   pass m     onesweep over byte m; each stripe is its own look-back chain
-//              (the stripe counts describe the input, which this first pass
-//              reads); afterwards every key sits in its byte-m bucket
-//   seghist    one read: per-bucket histograms of bytes 0..m-1
-//   segplan    per-bucket output offsets of bytes 0..m-1
-//   sched      round-robin order of the bucket tiles (tile j of every bucket
-//              before tile j+1 of any), so consecutive workgroups work on
-//              different buckets and each bucket's look-back chain only
-//              ever waits on the previous tile
-//   pass b     b = 0..m-1 (non-trivial only): LSD onesweep inside buckets;
-//              every later pass only permutes within buckets, so the bucket
-//              counts stay valid
-//   fixup      copies the result into keys_out if the pass count left it in
-//              the scratch buffer
+//   init      zero histograms and tile counters, seed the OR/AND reductions
+//   upfront   one read: histogram of byte 0 of the encoded keys, bitwise OR
+//             and AND of all encoded keys (byte b is trivial -- one value for
+//             every key -- iff OR and AND agree on it); clears descriptors
+//   pass p    p = 0..P-1, persistent workgroups; a trivial byte returns at
+//             once.  Each workgroup takes tiles from an atomic counter in
+//             order, prefetches the next tile's keys while it ranks the
+//             current one, looks back for its digit offsets, sorts the tile
+//             in LDS, writes digit runs with consecutive lanes on consecutive
+//             addresses and counts the digits of the next non-trivial byte on
+//             the way out (the next pass's histogram, flushed once per
+//             workgroup)
+//   fixup     copies the result into keys_out when the pass count left it in
+//             the scratch buffer
 // ---------------------------------------------------------------------------
 
-struct Stripes {
-  uint32_t count;      // S
-  uint32_t tiles_per;  // tiles per stripe (the last stripe may be shorter)
-  uint32_t num_tiles;
-  uint32_t tile_keys;
-};
-
-struct SortCtl {
-  int32_t msd;          // MSD byte, -1 when every byte is trivial
-  int32_t na;           // active passes: MSD pass + non-trivial lower bytes
-  uint32_t seg_tiles;   // tiles of all bucket chains
-  uint32_t seg_blocks;  // seghist workgroups
-};
-
-constexpr int kSegHistTiles = 32;  // tiles per seghist workgroup
-
-__global__ void init_kernel(uint32_t* __restrict__ a, uint32_t a_words, uint32_t* __restrict__ b, uint32_t b_words,
-                            uint32_t* __restrict__ counters) {
+__global__ void init_kernel(uint32_t* __restrict__ hist, uint32_t words, uint32_t* __restrict__ counters,
+                            uint64_t* __restrict__ red) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t st = gridDim.x * blockDim.x;
-  for (uint32_t i = g; i < a_words; i += st) a[i] = 0;
-  for (uint32_t i = g; i < b_words; i += st) b[i] = 0;
-  if (g < kMaxPasses + 1) counters[g] = 0;
+  for (uint32_t i = g; i < words; i += gridDim.x * blockDim.x) hist[i] = 0;
+  if (g < kMaxPasses) counters[g] = 0;
+  if (g == 0) {
+    red[0] = 0;     // OR identity
+    red[1] = ~0ull;  // AND identity
+  }
 }
 
-// Per-stripe histograms of every byte.  Grid = S * G workgroups, G per
-// stripe.  Also clears the look-back descriptors (desc_words 32-bit words).
-// hist layout: [byte][stripe][digit]
+template <typename Bits>
+__device__ __forceinline__ Bits wave_or(Bits v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, kWave);
+  return v;
+}
+template <typename Bits>
+__device__ __forceinline__ Bits wave_and(Bits v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v &= __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Byte-0 histogram plus OR/AND of the encoded keys; also clears the look-back
+// descriptors (desc_words 32-bit words).
 template <int KT, int BLOCK, bool VEC>
-__global__ __launch_bounds__(BLOCK) void hist_kernel(const typename Codec<KT>::Bits* __restrict__ keys, uint32_t n,
-                                                     Stripes st, uint32_t per_stripe_blocks,
-                                                     uint32_t* __restrict__ hist, uint32_t* __restrict__ desc,
-                                                     size_t desc_words) {
+__global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>::Bits* __restrict__ keys, uint32_t n,
+                                                        uint32_t* __restrict__ hist, uint64_t* __restrict__ red,
+                                                        uint32_t* __restrict__ desc, size_t desc_words) {
   using Bits = typename Codec<KT>::Bits;
-  constexpr int P = (int)sizeof(Bits);
-  __shared__ uint32_t h[P][kRadix];
+  __shared__ uint32_t h[kRadix];
+  __shared__ Bits s_or[BLOCK / kWave], s_and[BLOCK / kWave];
   const int tid = threadIdx.x;
-  for (int i = tid; i < P * kRadix; i += BLOCK) (&h[0][0])[i] = 0;
+  for (int i = tid; i < kRadix; i += BLOCK) h[i] = 0;
+  const size_t gtid = (size_t)blockIdx.x * BLOCK + tid;
+  const size_t gstride = (size_t)gridDim.x * BLOCK;
   {
-    const size_t gtid = (size_t)blockIdx.x * BLOCK + tid;
-    const size_t gstride = (size_t)gridDim.x * BLOCK;
     uint4* d4 = reinterpret_cast<uint4*>(desc);
     const size_t n4 = desc_words / 4;
     for (size_t i = gtid; i < n4; i += gstride) d4[i] = make_uint4(0, 0, 0, 0);
     for (size_t i = n4 * 4 + gtid; i < desc_words; i += gstride) desc[i] = 0;
   }
   __syncthreads();
-
-  const uint32_t s = blockIdx.x / per_stripe_blocks;
-  const uint32_t gb = blockIdx.x % per_stripe_blocks;
-  const size_t lo = (size_t)s * st.tiles_per * st.tile_keys;
-  const size_t hi = std::min<size_t>(lo + (size_t)st.tiles_per * st.tile_keys, n);
+  Bits vor = 0, vand = ~Bits(0);
   auto count = [&](Bits k) {
-#pragma unroll
-    for (int p = 0; p < P; ++p) atomicAdd(&h[p][digit_of<KT>(k, 8 * p)], 1u);
+    const Bits e = Codec<KT>::enc(k);
+    vor |= e;
+    vand &= e;
+    atomicAdd(&h[(uint32_t)e & 0xFFu], 1u);
   };
-  const size_t t0 = (size_t)gb * BLOCK + tid;
-  const size_t ts = (size_t)per_stripe_blocks * BLOCK;
   if constexpr (VEC) {
-    // stripes start at multiples of the tile size: 16-byte alignment of
-    // `keys` carries over to every stripe
     constexpr int V = 16 / (int)sizeof(Bits);
     using Vec = typename std::conditional<sizeof(Bits) == 4, uint4, ulonglong2>::type;
-    const Vec* kv = reinterpret_cast<const Vec*>(keys + lo);
-    const size_t nv = (hi - lo) / V;
-    for (size_t i = t0; i < nv; i += ts) {
-      Vec v = kv[i];
+    const Vec* kv = reinterpret_cast<const Vec*>(keys);
+    const size_t nv = n / V;
+    for (size_t i = gtid; i < nv; i += gstride) {
+      const Vec v = kv[i];
       if constexpr (sizeof(Bits) == 4) {
         count(v.x); count(v.y); count(v.z); count(v.w);
       } else {
         count(v.x); count(v.y);
       }
     }
-    for (size_t i = lo + nv * V + t0; i < hi; i += ts) count(keys[i]);
+    for (size_t i = nv * V + gtid; i < n; i += gstride) count(keys[i]);
   } else {
-    for (size_t i = lo + t0; i < hi; i += ts) count(keys[i]);
+    for (size_t i = gtid; i < n; i += gstride) count(keys[i]);
+  }
+  vor = wave_or(vor);
+  vand = wave_and(vand);
+  if ((tid & (kWave - 1)) == 0) {
+    s_or[tid / kWave] = vor;
+    s_and[tid / kWave] = vand;
   }
   __syncthreads();
-  for (int i = tid; i < P * kRadix; i += BLOCK) {
-    const uint32_t c = (&h[0][0])[i];
-    const int p = i / kRadix, d = i % kRadix;
-    if (c) atomicAdd(&hist[((size_t)p * st.count + s) * kRadix + d], c);
-  }
-}
-
-// One workgroup of 1024 threads per byte: stripe output offsets (in place),
-// global digit totals and the trivial flag.
-__global__ __launch_bounds__(1024) void plan_kernel(uint32_t* __restrict__ hist, uint32_t n, uint32_t stripes,
-                                                    uint32_t* __restrict__ glob, int32_t* __restrict__ trivial) {
-  __shared__ uint32_t part[4][kRadix];
-  __shared__ uint32_t base[kRadix];
-  __shared__ uint32_t sums[4];
-  const int p = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int d = tid & (kRadix - 1), q = tid >> 8;  // thread (d, q): quarter q of the stripes
-  uint32_t* h = hist + (size_t)p * stripes * kRadix;
-  const uint32_t s0 = (stripes * q) / 4, s1 = (stripes * (q + 1)) / 4;
-  uint32_t sum = 0;
-#pragma unroll 8
-  for (uint32_t s = s0; s < s1; ++s) sum += h[(size_t)s * kRadix + d];
-  part[q][d] = sum;
-  __syncthreads();
-  const uint32_t total = part[0][d] + part[1][d] + part[2][d] + part[3][d];
-  const int t = __syncthreads_or(q == 0 && total == n);
-  const uint32_t excl = block_excl_scan_256(q == 0 ? total : 0u, sums, tid);
-  if (q == 0) {
-    base[d] = excl;
-    glob[p * kRadix + d] = total;
-  }
-  __syncthreads();
-  uint32_t run = base[d];
-  for (int i = 0; i < q; ++i) run += part[i][d];
-#pragma unroll 8
-  for (uint32_t s = s0; s < s1; ++s) {
-    const uint32_t c = h[(size_t)s * kRadix + d];
-    h[(size_t)s * kRadix + d] = run;
-    run += c;
-  }
-  if (tid == 0) trivial[p] = t;
-}
-
-// 256 threads: MSD byte, bucket layout, tile and seghist-block prefixes.
-__global__ __launch_bounds__(256) void msd_kernel(const uint32_t* __restrict__ glob,
-                                                  const int32_t* __restrict__ trivial, int passes, uint32_t tile_keys,
-                                                  SortCtl* __restrict__ ctl, uint32_t* __restrict__ seg_off,
-                                                  uint32_t* __restrict__ tile_start, uint32_t* __restrict__ blk_start) {
-  __shared__ uint32_t sums[4];
-  const int tid = threadIdx.x;
-  int m = -1;
-  for (int p = passes - 1; p >= 0; --p)
-    if (!trivial[p]) {
-      m = p;
-      break;
+  if (tid == 0) {
+    Bits a = 0, b = ~Bits(0);
+    for (int i = 0; i < BLOCK / kWave; ++i) {
+      a |= s_or[i];
+      b &= s_and[i];
     }
-  if (m < 0) {
-    if (tid == 0) *ctl = SortCtl{-1, 0, 0, 0};
-    return;  // uniform branch: every thread returns
+    atomicOr(reinterpret_cast<unsigned long long*>(&red[0]), (unsigned long long)a);
+    atomicAnd(reinterpret_cast<unsigned long long*>(&red[1]), (unsigned long long)b | (sizeof(Bits) == 4 ? 0xFFFFFFFF00000000ull : 0ull));
   }
-  int na = 1;
-  for (int p = 0; p < m; ++p) na += trivial[p] ? 0 : 1;
-  const uint32_t len = glob[m * kRadix + tid];
-  const uint32_t tiles = (len + tile_keys - 1) / tile_keys;
-  const uint32_t blocks = (tiles + kSegHistTiles - 1) / kSegHistTiles;
-  const uint32_t off = block_excl_scan_256(len, sums, tid);
-  __syncthreads();
-  const uint32_t tstart = block_excl_scan_256(tiles, sums, tid);
-  __syncthreads();
-  const uint32_t bstart = block_excl_scan_256(blocks, sums, tid);
-  seg_off[tid] = off;
-  tile_start[tid] = tstart;
-  blk_start[tid] = bstart;
-  if (tid == kRadix - 1) {
-    seg_off[kRadix] = off + len;
-    tile_start[kRadix] = tstart + tiles;
-    blk_start[kRadix] = bstart + blocks;
-    *ctl = SortCtl{m, na, tstart + tiles, bstart + blocks};
+  for (int i = tid; i < kRadix; i += BLOCK) {
+    const uint32_t c = h[i];
+    if (c) atomicAdd(&hist[i], c);
   }
 }
 
-// Place of active pass k in the ping-pong sequence.
+// Which bytes vary: bit b set iff byte b differs between keys.
+__device__ __forceinline__ uint32_t varying_bytes(const uint64_t* red, int passes) {
+  const uint64_t diff = red[0] ^ red[1];
+  uint32_t m = 0;
+  for (int b = 0; b < passes; ++b)
+    if ((diff >> (8 * b)) & 0xFFu) m |= 1u << b;
+  return m;
+}
+
+// Place of active pass k (of na) in the ping-pong sequence.
 __device__ __forceinline__ int dst_of(int k, int na, int aliased) {
   if (aliased) return (k % 2 == 0) ? (int)kBufAlt : (int)kBufOut;
   return ((na - 1 - k) % 2 == 0) ? (int)kBufOut : (int)kBufAlt;
@@ -201,7 +134,6 @@ __device__ __forceinline__ int dst_of(int k, int na, int aliased) {
 __device__ __forceinline__ int src_of(int k, int na, int aliased) {
   return k == 0 ? (int)kBufIn : dst_of(k - 1, na, aliased);
 }
-
 template <typename T>
 __device__ __forceinline__ const T* pick_src(int id, const T* in, const T* alt, const T* out) {
   return id == kBufIn ? in : (id == kBufAlt ? alt : out);
@@ -211,180 +143,247 @@ __device__ __forceinline__ T* pick_dst(int id, T* alt, T* out) {
   return id == kBufAlt ? alt : out;
 }
 
-// Index of the bucket holding position x of a prefix array (last i with
-// start[i] <= x), over 256 buckets.
-__device__ __forceinline__ uint32_t bucket_of(const uint32_t* start, uint32_t x) {
-  uint32_t lo = 0;
-#pragma unroll
-  for (uint32_t step = kRadix / 2; step > 0; step >>= 1)
-    if (start[lo + step] <= x) lo += step;
-  return lo;
-}
-
-// MSD pass over byte ctl->msd of the whole input, stripes as chains.
-template <int KT, typename V, int BLOCK, int IPT, typename D>
-__global__ __launch_bounds__(BLOCK) void msd_pass_kernel(
-    const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
-    const V* v_in, V* v_alt, V* v_out, uint32_t n, int aliased, Stripes st, const SortCtl* __restrict__ ctl,
-    const uint32_t* __restrict__ stripe_base, uint32_t* __restrict__ tile_counter, D* __restrict__ desc) {
-  using Bits = typename Codec<KT>::Bits;
-  constexpr int TILE = BLOCK * IPT;
-  const int m = ctl->msd;
-  if (m < 0) return;
-  const int na = ctl->na;
-  const int db = dst_of(0, na, aliased);
-  Bits* dst = pick_dst(db, k_alt, k_out);
-  V* vdst = nullptr;
-  if constexpr (kHasVal<V>) vdst = pick_dst(db, v_alt, v_out);
-  __shared__ uint32_t s_id;
-  if (threadIdx.x == 0) {
-    const uint32_t ids = st.count * st.tiles_per;
-    uint32_t g, t;
-    do {
-      g = atomicAdd(&tile_counter[0], 1u);
-      t = (g % st.count) * st.tiles_per + g / st.count;
-    } while (t >= st.num_tiles && g < ids);
-    s_id = g < ids ? g : 0xFFFFFFFFu;
-  }
-  __syncthreads();
-  const uint32_t g = s_id;
-  if (g == 0xFFFFFFFFu) return;
-  const uint32_t s = g % st.count, j = g / st.count;
-  const size_t chain0 = (size_t)s * st.tiles_per * TILE;
-  const uint32_t chain_n = (uint32_t)std::min<size_t>((size_t)st.tiles_per * TILE, n - chain0);
-  const V* vs = nullptr;
-  if constexpr (kHasVal<V>) vs = v_in + chain0;
-  onesweep_tile<KT, V, BLOCK, IPT, D>(k_in + chain0, dst, vs, vdst, chain_n, n, j,
-                                      desc + (size_t)s * st.tiles_per * kRadix,
-                                      stripe_base + ((size_t)m * st.count + s) * kRadix, m * 8, 0);
-}
-
-// Per-bucket histograms of bytes 0..m-1 of the MSD pass output.
-// seg_hist layout: [bucket][byte][digit]
+// Histogram of byte `b` over keys[0, n), added into hist (grid-stride).
 template <int KT, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void seghist_kernel(const typename Codec<KT>::Bits* k_in,
-                                                        const typename Codec<KT>::Bits* k_alt,
-                                                        const typename Codec<KT>::Bits* k_out, int aliased,
-                                                        uint32_t tile_keys, const SortCtl* __restrict__ ctl,
-                                                        const uint32_t* __restrict__ seg_off,
-                                                        const uint32_t* __restrict__ blk_start,
-                                                        uint32_t* __restrict__ seg_hist) {
+__device__ void histogram_only(const typename Codec<KT>::Bits* keys, uint32_t n, int b, uint32_t* hist) {
+  __shared__ uint32_t h[kRadix];
+  for (int i = threadIdx.x; i < kRadix; i += BLOCK) h[i] = 0;
+  __syncthreads();
+  for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK)
+    atomicAdd(&h[digit_of<KT>(keys[i], 8 * b)], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kRadix; i += BLOCK)
+    if (h[i]) atomicAdd(&hist[b * kRadix + i], h[i]);
+}
+
+template <int KT, typename V, int IPT>
+__device__ __forceinline__ void load_tile(const typename Codec<KT>::Bits* __restrict__ src,
+                                          const V* __restrict__ vsrc, uint32_t n, uint32_t wb, bool full,
+                                          typename Codec<KT>::Bits (&k)[IPT],
+                                          typename std::conditional<kHasVal<V>, V, uint32_t>::type (&v)[IPT]) {
   using Bits = typename Codec<KT>::Bits;
-  constexpr int P = (int)sizeof(Bits);
-  __shared__ uint32_t h[P - 1][kRadix];
-  __shared__ uint32_t sb[kRadix + 1];
-  const int m = ctl->msd;
-  if (m <= 0 || blockIdx.x >= ctl->seg_blocks) return;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < (P - 1) * kRadix; i += BLOCK) (&h[0][0])[i] = 0;
-  for (int i = tid; i <= kRadix; i += BLOCK) sb[i] = blk_start[i];
-  __syncthreads();
-  const Bits* src = pick_src(dst_of(0, ctl->na, aliased), k_in, k_alt, k_out);
-  const uint32_t c = bucket_of(sb, blockIdx.x);
-  const uint32_t chunk = (uint32_t)kSegHistTiles * tile_keys;
-  const uint32_t b0 = seg_off[c] + (blockIdx.x - sb[c]) * chunk;
-  const uint32_t b1 = min(b0 + chunk, seg_off[c + 1]);
-  for (uint32_t i = b0 + tid; i < b1; i += BLOCK) {
-    const Bits k = src[i];
-    for (int p = 0; p < m; ++p) atomicAdd(&h[p][digit_of<KT>(k, 8 * p)], 1u);
-  }
-  __syncthreads();
-  for (int i = tid; i < m * kRadix; i += BLOCK) {
-    const uint32_t v = (&h[0][0])[i];
-    if (v) atomicAdd(&seg_hist[((size_t)c * (P - 1)) * kRadix + i], v);
+  if (full) {
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) k[i] = src[wb + i * kWave];
+    if constexpr (kHasVal<V>) {
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) v[i] = vsrc[wb + i * kWave];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const uint32_t idx = wb + i * kWave;
+      k[i] = idx < n ? src[idx] : Bits(0);
+      if constexpr (kHasVal<V>) v[i] = idx < n ? vsrc[idx] : V(0);
+    }
   }
 }
 
-// One workgroup per bucket: output offsets of bytes 0..m-1, in place.
-template <int P>
-__global__ __launch_bounds__(256) void segplan_kernel(uint32_t* seg_hist, const SortCtl* __restrict__ ctl,
-                                                      const uint32_t* __restrict__ seg_off) {
-  __shared__ uint32_t sums[4];
-  const int m = ctl->msd;
-  const uint32_t c = blockIdx.x;
-  const int tid = threadIdx.x;
-  const uint32_t base = seg_off[c];
-  for (int p = 0; p < m; ++p) {
-    uint32_t* row = seg_hist + ((size_t)c * (P - 1) + p) * kRadix;
-    const uint32_t v = row[tid];
-    const uint32_t e = block_excl_scan_256(v, sums, tid);
-    __syncthreads();
-    row[tid] = base + e;
-  }
-}
+#ifdef MGS_PHASE_PROBE
+__device__ uint64_t* g_phase_probe;
+#define PHASE(slot)                                                                                 \
+  do {                                                                                              \
+    if (g_phase_probe && tid == 0 && (tile & 63) == 0)                                              \
+      g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 8 + (slot)] = __builtin_amdgcn_s_memrealtime(); \
+  } while (0)
+#else
+#define PHASE(slot) \
+  do {              \
+  } while (0)
+#endif
 
-// Round-robin schedule of the bucket tiles: position of tile j of bucket c =
-// sum over buckets of min(j, tiles) + number of lower buckets with > j tiles.
-__global__ __launch_bounds__(256) void sched_kernel(const SortCtl* __restrict__ ctl,
-                                                    const uint32_t* __restrict__ tile_start,
-                                                    uint32_t* __restrict__ sched) {
-  __shared__ uint32_t ts[kRadix + 1];
-  __shared__ uint32_t tc[kRadix];
-  if (ctl->msd <= 0) return;
-  const int tid = threadIdx.x;
-  for (int i = tid; i <= kRadix; i += blockDim.x) ts[i] = tile_start[i];
-  __syncthreads();
-  for (int i = tid; i < kRadix; i += blockDim.x) tc[i] = ts[i + 1] - ts[i];
-  __syncthreads();
-  const uint32_t t = blockIdx.x * blockDim.x + tid;
-  if (t >= ctl->seg_tiles) return;
-  const uint32_t c = bucket_of(ts, t);
-  const uint32_t j = t - ts[c];
-  uint32_t pos = 0;
-  for (uint32_t i = 0; i < kRadix; ++i) {
-    const uint32_t ti = tc[i];
-    pos += min(j, ti) + ((i < c && ti > j) ? 1u : 0u);
-  }
-  sched[pos] = t;
-}
-
-// LSD pass over byte `pass` (< msd) inside the buckets.
+// ---------------------------------------------------------------------------
+// One onesweep pass over byte `pass`, persistent workgroups.
+// ---------------------------------------------------------------------------
 template <int KT, typename V, int BLOCK, int IPT, typename D>
-__global__ __launch_bounds__(BLOCK) void lsd_pass_kernel(
+__global__ __launch_bounds__(BLOCK) void pass_kernel(
     const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
-    const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, const SortCtl* __restrict__ ctl,
-    const int32_t* __restrict__ trivial, const uint32_t* __restrict__ seg_off,
-    const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ sched,
-    const uint32_t* __restrict__ seg_base, uint32_t* __restrict__ tile_counter, D* __restrict__ desc) {
+    const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, uint32_t num_tiles,
+    const uint64_t* __restrict__ red, uint32_t* __restrict__ hist, uint32_t* __restrict__ tile_counter,
+    D* __restrict__ desc) {
   using Bits = typename Codec<KT>::Bits;
   constexpr int P = (int)sizeof(Bits);
-  const int m = ctl->msd;
-  if (pass >= m || trivial[pass]) return;
-  int kb = 0;
-  for (int p = 0; p < pass; ++p) kb += trivial[p] ? 0 : 1;
-  const int k = 1 + kb, na = ctl->na;
+  constexpr int W = BLOCK / kWave;
+  constexpr int TILE = BLOCK * IPT;
+  constexpr bool HV = kHasVal<V>;
+  using VS = typename std::conditional<HV, V, uint32_t>::type;
+  static_assert(W >= 4, "need >= 256 threads for the per-digit phase");
+
+  const uint32_t vary = varying_bytes(red, P);
+  if (!(vary & (1u << pass))) {
+    // Trivial byte.  If it is the last trivial byte below the first varying
+    // one, nobody has counted that byte yet: do it here from the input.
+    const uint32_t below = vary & ((2u << pass) - 1u);
+    if (below == 0 && pass + 1 < P && (vary & (1u << (pass + 1))))
+      histogram_only<KT, BLOCK>(k_in, n, pass + 1, hist);
+    return;
+  }
+  const int k = __builtin_popcount(vary & ((1u << pass) - 1u));
+  const int na = __builtin_popcount(vary);
+  const uint32_t above = vary & ~((2u << pass) - 1u);
+  const int next = above ? __builtin_ctz(above) : -1;
   const int sb = src_of(k, na, aliased), db = dst_of(k, na, aliased);
-  const Bits* src = pick_src(sb, k_in, (const Bits*)k_alt, (const Bits*)k_out);
-  Bits* dst = pick_dst(db, k_alt, k_out);
-  const V* vsrc = nullptr;
-  V* vdst = nullptr;
-  if constexpr (kHasVal<V>) {
+  const Bits* __restrict__ src = pick_src(sb, k_in, (const Bits*)k_alt, (const Bits*)k_out);
+  Bits* __restrict__ dst = pick_dst(db, k_alt, k_out);
+  const V* __restrict__ vsrc = nullptr;
+  V* __restrict__ vdst = nullptr;
+  if constexpr (HV) {
     vsrc = pick_src(sb, v_in, (const V*)v_alt, (const V*)v_out);
     vdst = pick_dst(db, v_alt, v_out);
   }
-  __shared__ uint32_t s_t;
-  if (threadIdx.x == 0) {
-    const uint32_t g = atomicAdd(&tile_counter[1 + pass], 1u);
-    s_t = g < ctl->seg_tiles ? sched[g] : 0xFFFFFFFFu;
+  const int shift = 8 * pass, parity = k & 1;
+
+  struct RankSmem {
+    uint64_t peers[W][kRadix];
+  };
+  union StageSmem {
+    RankSmem r;
+    Bits keys[TILE];
+  };
+  __shared__ StageSmem u;
+  __shared__ VS vstage[HV ? TILE : 1];
+  __shared__ uint32_t off[W][kRadix];
+  __shared__ uint32_t gbase[kRadix];
+  __shared__ uint32_t base[kRadix];
+  __shared__ uint32_t nh[kRadix];
+  __shared__ uint32_t sums[4];
+  __shared__ uint32_t s_tile;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int w = tid / kWave;
+
+  {  // global digit bases of this byte
+    const uint32_t c = tid < kRadix ? hist[pass * kRadix + tid] : 0u;
+    const uint32_t e = block_excl_scan_256(c, sums, tid);
+    if (tid < kRadix) {
+      base[tid] = e;
+      nh[tid] = 0;
+    }
+    if (tid == 0) s_tile = atomicAdd(&tile_counter[pass], 1u);
+    __syncthreads();
   }
-  __syncthreads();
-  const uint32_t t = s_t;
-  if (t == 0xFFFFFFFFu) return;
-  const uint32_t c = bucket_of(tile_start, t);
-  const uint32_t j = t - tile_start[c];
-  const uint32_t b0 = seg_off[c], len = seg_off[c + 1] - b0;
-  const V* vs = nullptr;
-  if constexpr (kHasVal<V>) vs = vsrc + b0;
-  onesweep_tile<KT, V, BLOCK, IPT, D>(src + b0, dst, vs, vdst, len, n, j, desc + (size_t)tile_start[c] * kRadix,
-                                      seg_base + ((size_t)c * (P - 1) + pass) * kRadix, pass * 8, kb & 1);
+  uint32_t tile = s_tile;
+  Bits kc[IPT];
+  VS vc[IPT];
+  if (tile < num_tiles) {
+    const uint32_t tb = tile * (uint32_t)TILE;
+    load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kc, vc);
+  }
+
+  while (tile < num_tiles) {
+    PHASE(0);
+    for (int i = tid; i < W * kRadix; i += BLOCK) {
+      (&u.r.peers[0][0])[i] = 0;
+      (&off[0][0])[i] = 0;
+    }
+    __syncthreads();
+    PHASE(1);
+
+    const uint32_t tile_base = tile * (uint32_t)TILE;
+    const uint32_t tile_cnt = min((uint32_t)TILE, n - tile_base);
+    const bool full = tile_cnt == (uint32_t)TILE;
+    const uint32_t wb = tile_base + (uint32_t)(w * IPT * kWave + lane);
+    uint32_t dg[IPT], rk[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) dg[i] = digit_of<KT>(kc[i], shift);
+    uint64_t* pt = u.r.peers[w];
+    uint32_t* ct = off[w];
+    (void)pt;
+    if (full) {
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) rk[i] = MGS_RANK(dg[i], pt, ct);
+    } else {
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+        if (wb + i * kWave < n) rk[i] = MGS_RANK(dg[i], pt, ct);
+    }
+    __syncthreads();
+    PHASE(2);
+
+    uint32_t total = 0;
+    D* my_desc = desc + (size_t)tile * kRadix + tid;
+    if (tid < kRadix) {
+#pragma unroll
+      for (int i = 0; i < W; ++i) total += off[i][tid];
+      st_agent(my_desc, Desc<D>::make(tile == 0 ? kStInclusive : kStAggregate, (D)total, parity));
+    }
+    const uint32_t excl = block_excl_scan_256(total, sums, tid);
+    if (tid < kRadix) {
+      uint32_t run = excl;
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const uint32_t c = off[i][tid];
+        off[i][tid] = run;
+        run += c;
+      }
+      uint32_t prefix = 0;
+      if (tile > 0) {
+        prefix = lookback(desc, tile, (uint32_t)tid, parity);
+        st_agent(my_desc, Desc<D>::make(kStInclusive, (D)(prefix + total), parity));
+      }
+      gbase[tid] = base[tid] + prefix - excl;
+      if (tid == 0) PHASE(4);
+    }
+    __syncthreads();
+    PHASE(3);
+    // This tile's inclusive prefixes are published: only now take the next
+    // tile, so a tile's predecessor is always one that is already being
+    // ranked (taking it earlier makes look-backs wait on tiles nobody works
+    // on yet).
+    if (tid == 0) s_tile = atomicAdd(&tile_counter[pass], 1u);
+
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const uint32_t pos = off[w][dg[i]] + rk[i];
+      if ((full || wb + i * kWave < n) && pos < (uint32_t)TILE) {
+        u.keys[pos] = kc[i];
+        if constexpr (HV) vstage[pos] = vc[i];
+      }
+    }
+    __syncthreads();
+    const uint32_t nxt = s_tile;
+
+    // prefetch the next tile while this one is written out
+    Bits kn[IPT];
+    VS vn[IPT];
+    if (nxt < num_tiles) {
+      const uint32_t tb = nxt * (uint32_t)TILE;
+      load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kn, vn);
+    }
+
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t i = (uint32_t)(j * BLOCK + tid);
+      if (full || i < tile_cnt) {
+        const Bits kk = u.keys[i];
+        const uint32_t o = gbase[digit_of<KT>(kk, shift)] + i;
+        if (o < n) {
+          dst[o] = kk;
+          if constexpr (HV) vdst[o] = vstage[i];
+        }
+        if (next >= 0) atomicAdd(&nh[digit_of<KT>(kk, 8 * next)], 1u);
+      }
+    }
+    PHASE(5);
+    __syncthreads();
+    PHASE(6);
+    tile = nxt;
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      kc[i] = kn[i];
+      if constexpr (HV) vc[i] = vn[i];
+    }
+  }
+  if (next >= 0 && tid < kRadix && nh[tid]) atomicAdd(&hist[next * kRadix + tid], nh[tid]);
 }
 
 // Copies the result into the output buffers when the passes left it in the
 // scratch buffer (or did not run).
 template <typename Bits, typename V>
 __global__ void fixup_kernel(const Bits* k_in, const Bits* k_alt, Bits* k_out, const V* v_in, const V* v_alt, V* v_out,
-                             uint32_t n, const SortCtl* __restrict__ ctl, int aliased) {
-  const int na = ctl->na;
+                             uint32_t n, const uint64_t* __restrict__ red, int aliased) {
+  const int na = __builtin_popcount(varying_bytes(red, (int)sizeof(Bits)));
   int s;
   if (na == 0) {
     s = kBufIn;
@@ -422,30 +421,18 @@ namespace {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-#ifndef MGS_STRIPES
-#define MGS_STRIPES 256
-#endif
-
 struct SortLayout {
-  size_t alt_k, alt_v, hist, glob, trivial, ctl, seg_off, tile_start, blk_start, seg_hist, sched, counters, desc_a,
-      desc_b, total;
-  Stripes st;
-  uint32_t seg_tiles_ub;
+  size_t alt_k, alt_v, hist, red, counters, desc, total;
+  uint32_t num_tiles;
   bool wide_desc;
 };
 
 template <int KB, int VB>
 SortLayout sort_layout(size_t n) {
   using Cfg = TileCfg<KB, VB>;
-  constexpr int P = KB;
   SortLayout L{};
-  L.st.tile_keys = Cfg::kTile;
-  L.st.num_tiles = (uint32_t)((n + Cfg::kTile - 1) / Cfg::kTile);
-  L.st.tiles_per = (L.st.num_tiles + MGS_STRIPES - 1) / MGS_STRIPES;
-  L.st.count = (L.st.num_tiles + L.st.tiles_per - 1) / L.st.tiles_per;
-  L.seg_tiles_ub = L.st.num_tiles + kRadix;
+  L.num_tiles = (uint32_t)((n + Cfg::kTile - 1) / Cfg::kTile);
   L.wide_desc = n > (size_t(1) << 30);
-  const size_t dw = L.wide_desc ? 8 : 4;
   size_t o = 0;
   auto take = [&](size_t bytes) {
     const size_t at = o;
@@ -454,25 +441,38 @@ SortLayout sort_layout(size_t n) {
   };
   L.alt_k = take(n * KB);
   L.alt_v = take(n * VB);
-  L.hist = take((size_t)P * L.st.count * kRadix * 4);
-  L.glob = take((size_t)P * kRadix * 4);
-  L.trivial = take(kMaxPasses * 4);
-  L.ctl = take(sizeof(SortCtl));
-  L.seg_off = take((kRadix + 1) * 4);
-  L.tile_start = take((kRadix + 1) * 4);
-  L.blk_start = take((kRadix + 1) * 4);
-  L.seg_hist = take((size_t)kRadix * (P - 1) * kRadix * 4);
-  L.sched = take((size_t)L.seg_tiles_ub * 4);
-  L.counters = take((kMaxPasses + 1) * 4);
-  L.desc_a = take((size_t)L.st.num_tiles * kRadix * dw);  // must stay adjacent:
-  L.desc_b = take((size_t)L.seg_tiles_ub * kRadix * dw);  // cleared as one range
+  L.hist = take((size_t)KB * kRadix * 4);
+  L.red = take(2 * sizeof(uint64_t));
+  L.counters = take(kMaxPasses * 4);
+  L.desc = take((size_t)L.num_tiles * kRadix * (L.wide_desc ? 8 : 4));
   L.total = o;
   return L;
 }
 
+// Workgroups of `kernel` that fit on the device at once (persistent grid).
+template <typename K>
+uint32_t resident_blocks(K kernel, int block, size_t dyn_lds) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, dyn_lds) != hipSuccess) return 0;
+  return (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
+}
+
+// Records events[i] (when given) before stage i; stage boundaries are the
+// kernel launches of the sequence below.
+struct Marks {
+  hipEvent_t* ev;
+  int cap, used;
+  void operator()(hipStream_t s) {
+    if (ev != nullptr && used < cap) (void)hipEventRecord(ev[used], s);
+    ++used;
+  }
+};
+
 template <int KT, int VT>
 int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t n, void* temp, size_t temp_bytes,
-               hipStream_t s) {
+               hipStream_t s, Marks& mark) {
   using Bits = typename Codec<KT>::Bits;
   using V = typename ValOf<VT>::T;
   constexpr int KB = sizeof(Bits);
@@ -488,7 +488,9 @@ int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t 
   V* vo = static_cast<V*>(vout);
 
   if (n <= (size_t)SCfg::kTile) {
+    mark(s);
     small_sort_kernel<KT, V, SCfg::kBlock, SCfg::kIpt><<<1, SCfg::kBlock, 0, s>>>(ki, ko, vi, vo, (uint32_t)n);
+    mark(s);
     return check_launch();
   }
 
@@ -498,72 +500,52 @@ int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t 
   Bits* ka = reinterpret_cast<Bits*>(t + L.alt_k);
   V* va = reinterpret_cast<V*>(reinterpret_cast<VP*>(t + L.alt_v));
   uint32_t* hist = reinterpret_cast<uint32_t*>(t + L.hist);
-  uint32_t* glob = reinterpret_cast<uint32_t*>(t + L.glob);
-  int32_t* trivial = reinterpret_cast<int32_t*>(t + L.trivial);
-  SortCtl* ctl = reinterpret_cast<SortCtl*>(t + L.ctl);
-  uint32_t* seg_off = reinterpret_cast<uint32_t*>(t + L.seg_off);
-  uint32_t* tile_start = reinterpret_cast<uint32_t*>(t + L.tile_start);
-  uint32_t* blk_start = reinterpret_cast<uint32_t*>(t + L.blk_start);
-  uint32_t* seg_hist = reinterpret_cast<uint32_t*>(t + L.seg_hist);
-  uint32_t* sched = reinterpret_cast<uint32_t*>(t + L.sched);
+  uint64_t* red = reinterpret_cast<uint64_t*>(t + L.red);
   uint32_t* counters = reinterpret_cast<uint32_t*>(t + L.counters);
-  void* desc_a = t + L.desc_a;
-  void* desc_b = t + L.desc_b;
-  const size_t desc_words = (L.total - L.desc_a) / 4;
+  void* desc = t + L.desc;
+  const size_t desc_words = (size_t)L.num_tiles * kRadix * (L.wide_desc ? 2 : 1);
   const int aliased = (kin == kout) || (kHasVal<V> && vin == vout);
-  const uint32_t hist_words = (uint32_t)(P * L.st.count * kRadix);
-  const uint32_t seg_words = (uint32_t)(kRadix * (P - 1) * kRadix);
 
-  init_kernel<<<64, 1024, 0, s>>>(hist, hist_words, seg_hist, seg_words, counters);
+  mark(s);
+  init_kernel<<<1, 1024, 0, s>>>(hist, (uint32_t)(P * kRadix), counters, red);
+  mark(s);
   {
     constexpr int HB = 256;
-    const size_t stripe_keys = (size_t)L.st.tiles_per * L.st.tile_keys;
-    const uint32_t g = (uint32_t)std::max<size_t>(
-        1, std::min<size_t>(2048 / L.st.count, (stripe_keys + 16 * 1024 - 1) / (16 * 1024)));
-    const unsigned grid = L.st.count * g;
+    const unsigned grid = (unsigned)std::min<size_t>(std::max<size_t>((n + 16 * HB - 1) / (16 * HB), 1), 2048);
     if ((reinterpret_cast<uintptr_t>(kin) % 16) == 0)
-      hist_kernel<KT, HB, true><<<grid, HB, 0, s>>>(ki, (uint32_t)n, L.st, g, hist, (uint32_t*)desc_a, desc_words);
+      upfront_kernel<KT, HB, true><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words);
     else
-      hist_kernel<KT, HB, false><<<grid, HB, 0, s>>>(ki, (uint32_t)n, L.st, g, hist, (uint32_t*)desc_a, desc_words);
+      upfront_kernel<KT, HB, false><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words);
   }
-  plan_kernel<<<P, 1024, 0, s>>>(hist, (uint32_t)n, L.st.count, glob, trivial);
-  msd_kernel<<<1, 256, 0, s>>>(glob, trivial, P, Cfg::kTile, ctl, seg_off, tile_start, blk_start);
-  if (L.wide_desc)
-    msd_pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint64_t><<<L.st.num_tiles, Cfg::kBlock, 0, s>>>(
-        ki, ka, ko, vi, va, vo, (uint32_t)n, aliased, L.st, ctl, hist, counters, (uint64_t*)desc_a);
-  else
-    msd_pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint32_t><<<L.st.num_tiles, Cfg::kBlock, 0, s>>>(
-        ki, ka, ko, vi, va, vo, (uint32_t)n, aliased, L.st, ctl, hist, counters, (uint32_t*)desc_a);
-  {
-    const unsigned grid = (unsigned)(L.seg_tiles_ub / kSegHistTiles + kRadix + 1);
-    seghist_kernel<KT, 256><<<grid, 256, 0, s>>>(ki, ka, ko, aliased, Cfg::kTile, ctl, seg_off, blk_start, seg_hist);
-  }
-  segplan_kernel<P><<<kRadix, 256, 0, s>>>(seg_hist, ctl, seg_off);
-  sched_kernel<<<(L.seg_tiles_ub + 255) / 256, 256, 0, s>>>(ctl, tile_start, sched);
-  for (int p = 0; p < P - 1; ++p) {
+  auto launch_pass = [&](auto kernel, auto* dsc, int p) {
+    const uint32_t res = resident_blocks(kernel, Cfg::kBlock, 0);
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(L.num_tiles, res ? res : L.num_tiles));
+    kernel<<<grid, Cfg::kBlock, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, p, aliased, L.num_tiles, red, hist,
+                                        counters, dsc);
+  };
+  for (int p = 0; p < P; ++p) {
+    mark(s);
     if (L.wide_desc)
-      lsd_pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint64_t><<<L.seg_tiles_ub, Cfg::kBlock, 0, s>>>(
-          ki, ka, ko, vi, va, vo, (uint32_t)n, p, aliased, ctl, trivial, seg_off, tile_start, sched, seg_hist,
-          counters, (uint64_t*)desc_b);
+      launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint64_t>, (uint64_t*)desc, p);
     else
-      lsd_pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint32_t><<<L.seg_tiles_ub, Cfg::kBlock, 0, s>>>(
-          ki, ka, ko, vi, va, vo, (uint32_t)n, p, aliased, ctl, trivial, seg_off, tile_start, sched, seg_hist,
-          counters, (uint32_t*)desc_b);
+      launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint32_t>, (uint32_t*)desc, p);
   }
+  mark(s);
   {
     const int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
-    fixup_kernel<Bits, V><<<grid, 256, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, ctl, aliased);
+    fixup_kernel<Bits, V><<<grid, 256, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, red, aliased);
   }
+  mark(s);
   return check_launch();
 }
 
 template <int KT>
 int sort_by_val(mgs_val vt, const void* kin, void* kout, const void* vin, void* vout, size_t n, void* temp,
-                size_t temp_bytes, hipStream_t s) {
+                size_t temp_bytes, hipStream_t s, Marks& m) {
   switch (vt) {
-    case MGS_VAL_NONE: return sort_typed<KT, MGS_VAL_NONE>(kin, kout, vin, vout, n, temp, temp_bytes, s);
-    case MGS_VAL_U32: return sort_typed<KT, MGS_VAL_U32>(kin, kout, vin, vout, n, temp, temp_bytes, s);
-    case MGS_VAL_U64: return sort_typed<KT, MGS_VAL_U64>(kin, kout, vin, vout, n, temp, temp_bytes, s);
+    case MGS_VAL_NONE: return sort_typed<KT, MGS_VAL_NONE>(kin, kout, vin, vout, n, temp, temp_bytes, s, m);
+    case MGS_VAL_U32: return sort_typed<KT, MGS_VAL_U32>(kin, kout, vin, vout, n, temp, temp_bytes, s, m);
+    case MGS_VAL_U64: return sort_typed<KT, MGS_VAL_U64>(kin, kout, vin, vout, n, temp, temp_bytes, s, m);
   }
   return MGS_ERR_BAD_TYPE;
 }
@@ -589,16 +571,26 @@ size_t sort_temp_bytes(mgs_key kt, mgs_val vt, size_t n) {
 }
 
 int sort_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout, size_t n,
-                  void* temp, size_t temp_bytes, hipStream_t s) {
+                  void* temp, size_t temp_bytes, hipStream_t s, hipEvent_t* events, int num_events,
+                  int* num_marks) {
+  Marks m{events, num_events, 0};
+  int r = MGS_ERR_BAD_TYPE;
   switch (kt) {
-    case MGS_KEY_U32: return sort_by_val<MGS_KEY_U32>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s);
-    case MGS_KEY_U64: return sort_by_val<MGS_KEY_U64>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s);
-    case MGS_KEY_I32: return sort_by_val<MGS_KEY_I32>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s);
-    case MGS_KEY_I64: return sort_by_val<MGS_KEY_I64>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s);
-    case MGS_KEY_F32: return sort_by_val<MGS_KEY_F32>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s);
-    case MGS_KEY_F64: return sort_by_val<MGS_KEY_F64>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s);
+    case MGS_KEY_U32: r = sort_by_val<MGS_KEY_U32>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s, m); break;
+    case MGS_KEY_U64: r = sort_by_val<MGS_KEY_U64>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s, m); break;
+    case MGS_KEY_I32: r = sort_by_val<MGS_KEY_I32>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s, m); break;
+    case MGS_KEY_I64: r = sort_by_val<MGS_KEY_I64>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s, m); break;
+    case MGS_KEY_F32: r = sort_by_val<MGS_KEY_F32>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s, m); break;
+    case MGS_KEY_F64: r = sort_by_val<MGS_KEY_F64>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s, m); break;
   }
-  return MGS_ERR_BAD_TYPE;
+  if (num_marks) *num_marks = m.used;
+  return r;
 }
 
 }  // namespace mgs
+
+#ifdef MGS_PHASE_PROBE
+extern "C" int mgs_probe_set(uint64_t* p) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(mgs::g_phase_probe), &p, sizeof(p)) == hipSuccess ? 0 : -1;
+}
+#endif

@@ -1,0 +1,258 @@
+// sort_benchmark: the reference's CLI driver (src/sort_benchmark.cu:64-228) on
+// MI355X.  Same positional argument, --flag value pairs, algorithm names and
+// CSV columns:
+//   num_elements, algorithm, gpus, data_type, distribution_type,
+//   distribution_seed, num_threads, chunk_size, merge_group_size, num_buffers,
+//   memory_allocation_duration, sort_duration, memory_deallocation_duration,
+//   total_duration
+// followed by added columns:
+//   gkeys_per_s, alg_bytes, hbm_gbps_alg, roofline_frac, cpu_threads, verified
+// The run also verifies its output: sorted by the key order and a
+// permutation of the input (order-independent fingerprint); the reference
+// only checks std::is_sorted and still exits 0 (:127, :223-227) -- this
+// driver exits 1 on a failed check.
+#include <hip/hip_runtime.h>
+#include <omp.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <iomanip>
+#include <iostream>
+#include <parallel/algorithm>
+#include <string>
+#include <vector>
+
+#include "argument_limits.h"
+#include "data_generator.h"
+#include "device_buffer.h"
+#include "gowanlock_sort.h"
+#include "hip_check.h"
+#include "host_vector.h"
+#include "key_traits.h"
+#include "paradis.h"
+#include "tanasic_sort.h"
+#include "time_durations.h"
+
+using namespace mgsort;
+
+namespace {
+
+[[noreturn]] void PrintUsageAndExit() {
+  std::cout << ArgumentLimits::Usage();
+  std::exit(1);
+}
+
+[[noreturn]] void PrintErrorUsageAndExit(const std::string& error) {
+  std::cout << "Error: " << error << "\n\n";
+  PrintUsageAndExit();
+}
+
+bool IsOptionPresent(const std::vector<std::string>& a, const std::string& name) {
+  return std::find(a.begin(), a.end(), name) != a.end();
+}
+
+bool GetOptionValue(const std::vector<std::string>& a, const std::string& name, std::string* v) {
+  auto it = std::find(a.begin(), a.end(), name);
+  if (it != a.end() && it + 1 != a.end()) {
+    *v = *(it + 1);
+    return true;
+  }
+  return false;
+}
+
+struct Options {
+  size_t num_elements = 0;
+  std::string algorithm, data_type, distribution_type, values = "none", merge_backend = "cpu";
+  std::vector<int> gpus;
+  uint32_t seed = 0;
+  int num_threads = 1;
+  size_t chunk_size = 0, merge_group_size = 1, num_buffers = 2, partitions = 0;
+};
+
+// Order-independent fingerprint of a multiset of keys (sum and xor of a
+// mixed image of every key).
+template <typename T>
+std::pair<uint64_t, uint64_t> Fingerprint(const T* a, size_t n) {
+  uint64_t s = 0, x = 0;
+#pragma omp parallel for reduction(+ : s) reduction(^ : x)
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t v = (uint64_t)Encode(a[i]);
+    v ^= v >> 31;
+    v *= 0x9E3779B97F4A7C15ull;
+    v ^= v >> 29;
+    s += v;
+    x ^= v;
+  }
+  return {s, x};
+}
+
+template <typename T>
+bool IsSortedEncoded(const T* a, size_t n) {
+  bool ok = true;
+#pragma omp parallel for reduction(&& : ok)
+  for (size_t i = 1; i < n; ++i) ok = ok && !(Encode(a[i]) < Encode(a[i - 1]));
+  return ok;
+}
+
+// The "thrust" branch (reference :81-91): H2D, sort, D2H inside sort_phase.
+// The "hip" branch: in-HBM sort, copies outside the timed phase.
+template <typename T>
+void DeviceSort(HostVector<T>* e, const Options& o, bool include_copies) {
+  HIP_CHECK(hipSetDevice(o.gpus.front()));
+  HIP_CHECK(hipDeviceSynchronize());
+  const size_t n = e->size();
+  const mgs_key kt = KeyOf<T>::value;
+  const mgs_val vt = o.values == "u32" ? MGS_VAL_U32 : (o.values == "u64" ? MGS_VAL_U64 : MGS_VAL_NONE);
+  const size_t vb = vt == MGS_VAL_U32 ? 4 : (vt == MGS_VAL_U64 ? 8 : 0);
+  TimeDurations::Get()->Tic("memory_allocation");
+  DeviceBuffer keys(n * sizeof(T)), vals(n * vb), temp(mgs_sort_temp_bytes(kt, vt, n));
+  Stream s;
+  TimeDurations::Get()->Toc("memory_allocation");
+  if (vb) {  // payload = original index, so the output shows stability
+    std::vector<uint64_t> idx(n);
+    for (size_t i = 0; i < n; ++i) idx[i] = i;
+    if (vb == 4) {
+      std::vector<uint32_t> i32(idx.begin(), idx.end());
+      HIP_CHECK(hipMemcpy(vals.get(), i32.data(), n * 4, hipMemcpyHostToDevice));
+    } else {
+      HIP_CHECK(hipMemcpy(vals.get(), idx.data(), n * 8, hipMemcpyHostToDevice));
+    }
+  }
+  if (!include_copies) HIP_CHECK(hipMemcpy(keys.get(), e->data(), n * sizeof(T), hipMemcpyHostToDevice));
+  TimeDurations::Get()->Tic("sort_phase");
+  if (include_copies)
+    HIP_CHECK(hipMemcpyAsync(keys.get(), e->data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+  MGS_CHECK(mgs_sort(kt, vt, keys.get(), keys.get(), vals.get(), vals.get(), n, temp.get(), temp.bytes(), s));
+  if (include_copies)
+    HIP_CHECK(hipMemcpyAsync(e->data(), keys.get(), n * sizeof(T), hipMemcpyDeviceToHost, s));
+  s.Sync();
+  TimeDurations::Get()->Toc("sort_phase");
+  if (!include_copies) HIP_CHECK(hipMemcpy(e->data(), keys.get(), n * sizeof(T), hipMemcpyDeviceToHost));
+  TimeDurations::Get()->Tic("memory_deallocation");
+  keys.Free();
+  vals.Free();
+  temp.Free();
+  TimeDurations::Get()->Toc("memory_deallocation");
+}
+
+template <typename T>
+bool GenerateAndSortData(const Options& o) {
+  DataGenerator generator(o.seed);
+  HostVector<T> elements(o.num_elements);
+  if (!generator.ComputeDistribution<T>(elements.data(), elements.size(), o.distribution_type))
+    PrintErrorUsageAndExit("distribution " + o.distribution_type + " is not defined for " + o.data_type);
+  const auto before = Fingerprint(elements.data(), elements.size());
+  const std::string& a = o.algorithm;
+  if (a == "tanasic") {
+    TanasicSort<T>(&elements, o.gpus, o.partitions);
+  } else if (a == "gowanlock") {
+    SetupGowanlockSort(o.gpus, o.num_threads);
+    GowanlockSort<T>(&elements, o.chunk_size, o.merge_group_size, o.num_buffers, o.gpus, o.merge_backend);
+  } else if (a == "thrust" || a == "hip") {
+    DeviceSort<T>(&elements, o, a == "thrust");
+  } else if (a == "paradis") {
+    TimeDurations::Get()->Tic("sort_phase");
+    ParadisSort<T>(elements.data(), elements.data() + elements.size(), o.num_threads);
+    TimeDurations::Get()->Toc("sort_phase");
+  } else if (a == "gnu_parallel") {
+    omp_set_num_threads(o.num_threads);
+    TimeDurations::Get()->Tic("sort_phase");
+    __gnu_parallel::sort(elements.begin(), elements.end(), EncodedLess<T>());
+    TimeDurations::Get()->Toc("sort_phase");
+  } else if (a == "std") {
+    TimeDurations::Get()->Tic("sort_phase");
+    std::sort(elements.begin(), elements.end(), EncodedLess<T>());
+    TimeDurations::Get()->Toc("sort_phase");
+  } else if (a == "boost") {
+    std::cerr << "boost::sort::block_indirect_sort is not part of this build\n";
+    std::exit(2);
+  }
+  const bool ok = IsSortedEncoded(elements.data(), elements.size()) &&
+                  Fingerprint(elements.data(), elements.size()) == before;
+
+  auto* td = TimeDurations::Get();
+  std::cout << o.num_elements << ",\"" << a << "\",\"";
+  for (size_t i = 0; i < o.gpus.size(); ++i) std::cout << o.gpus[i] << (i + 1 < o.gpus.size() ? "," : "\"");
+  std::cout << ",\"" << o.data_type << "\",\"" << o.distribution_type << "\"," << o.seed << "," << o.num_threads << ","
+            << o.chunk_size << "," << o.merge_group_size << "," << o.num_buffers << ",";
+  std::cout << std::fixed << std::setprecision(9) << td->Seconds("memory_allocation") << ","
+            << td->Seconds("sort_phase") << "," << td->Seconds("memory_deallocation") << ","
+            << td->TotalExecutionTime().count() << ",";
+  // added columns: throughput against the fixed HBM roofline of an 8-bit
+  // LSD onesweep (one histogram read, one read and one write per byte)
+  const double t = td->Seconds("sort_phase");
+  const double kb = sizeof(T);
+  const double vb = o.values == "u32" ? 4 : (o.values == "u64" ? 8 : 0);
+  const double alg_bytes = o.num_elements * (kb + kb * 2 * (kb + vb));
+  const double gkeys = t > 0 ? o.num_elements / t / 1e9 : 0;
+  const double gbps = t > 0 ? alg_bytes / t / 1e9 : 0;
+  std::cout << std::setprecision(6) << gkeys << "," << (uint64_t)alg_bytes << "," << gbps << "," << gbps / 8000.0
+            << "," << omp_get_num_procs() << "," << (ok ? 1 : 0) << "\n";
+  return ok;
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+  const std::vector<std::string> args(argv + 1, argv + argc);
+  if (args.empty() || args.size() % 2 == 0 || IsOptionPresent(args, "--help")) PrintUsageAndExit();
+  Options o;
+  std::string v;
+  if (!ArgumentLimits::ParseNumElements(args.front(), &o.num_elements))
+    PrintErrorUsageAndExit("Invalid <num_elements> = " + args.front());
+  o.algorithm = ArgumentLimits::GetDefaultAlgorithm();
+  if (GetOptionValue(args, "--algorithm", &v) && !ArgumentLimits::ParseAlgorithm(v, &o.algorithm))
+    PrintErrorUsageAndExit("Invalid <algorithm> = " + v);
+  o.gpus = ArgumentLimits::GetDefaultGpus();
+  if (GetOptionValue(args, "--gpus", &v) && !ArgumentLimits::ParseGpus(v, &o.gpus))
+    PrintErrorUsageAndExit("Invalid <gpus> = " + v);
+  o.data_type = ArgumentLimits::GetDefaultDataType();
+  if (GetOptionValue(args, "--data_type", &v) && !ArgumentLimits::ParseDataType(v, &o.data_type))
+    PrintErrorUsageAndExit("Invalid <data_type> = " + v);
+  o.distribution_type = ArgumentLimits::GetDefaultDistributionType();
+  if (GetOptionValue(args, "--distribution_type", &v) &&
+      !ArgumentLimits::ParseDistributionType(v, &o.distribution_type))
+    PrintErrorUsageAndExit("Invalid <distribution_type> = " + v);
+  o.seed = ArgumentLimits::GetDefaultDistributionSeed();
+  if (GetOptionValue(args, "--distribution_seed", &v) && !ArgumentLimits::ParseDistributionSeed(v, &o.seed))
+    PrintErrorUsageAndExit("Invalid <distribution_seed> = " + v);
+  o.num_threads = ArgumentLimits::GetDefaultNumThreads();
+  if (GetOptionValue(args, "--num_threads", &v) && !ArgumentLimits::ParseNumThreads(v, &o.num_threads))
+    PrintErrorUsageAndExit("Invalid <num_threads> = " + v);
+  o.chunk_size = ArgumentLimits::GetDefaultChunkSize();
+  if (GetOptionValue(args, "--chunk_size", &v) && !ArgumentLimits::ParseChunkSize(v, &o.chunk_size))
+    PrintErrorUsageAndExit("Invalid <chunk_size> = " + v);
+  o.merge_group_size = ArgumentLimits::GetDefaultMergeGroupSize();
+  if (GetOptionValue(args, "--merge_group_size", &v) &&
+      !ArgumentLimits::ParseMergeGroupSize(v, &o.merge_group_size))
+    PrintErrorUsageAndExit("Invalid <merge_group_size> = " + v);
+  o.num_buffers = ArgumentLimits::GetDefaultNumBuffers();
+  if (GetOptionValue(args, "--num_buffers", &v) && !ArgumentLimits::ParseNumBuffers(v, &o.num_buffers))
+    PrintErrorUsageAndExit("Invalid <num_buffers> = " + v);
+  if (GetOptionValue(args, "--values", &v) && !ArgumentLimits::ParseValues(v, &o.values))
+    PrintErrorUsageAndExit("Invalid <values> = " + v);
+  if (GetOptionValue(args, "--partitions", &v) && !ArgumentLimits::ParsePartitions(v, &o.partitions))
+    PrintErrorUsageAndExit("Invalid <partitions> = " + v);
+  if (GetOptionValue(args, "--merge_backend", &v) && !ArgumentLimits::ParseMergeBackend(v, &o.merge_backend))
+    PrintErrorUsageAndExit("Invalid <merge_backend> = " + v);
+
+  bool ok = false;
+  try {
+    const std::string& t = o.data_type;
+    if (t == "int") ok = GenerateAndSortData<int32_t>(o);
+    else if (t == "long") ok = GenerateAndSortData<int64_t>(o);
+    else if (t == "float") ok = GenerateAndSortData<float>(o);
+    else if (t == "double") ok = GenerateAndSortData<double>(o);
+    else if (t == "u32") ok = GenerateAndSortData<uint32_t>(o);
+    else if (t == "u64") ok = GenerateAndSortData<uint64_t>(o);
+  } catch (const std::exception& ex) {
+    std::cout << "Error: " << ex.what() << "\n";
+    return 2;
+  }
+  if (!ok) {
+    std::cout << "Error: Invalid sort order.\n";
+    return 1;
+  }
+  return 0;
+}

@@ -28,7 +28,7 @@ VAL_BYTES = {VAL_NONE: 0, VAL_U32: 4, VAL_U64: 8}
 MGS_OK = 0
 
 SYMBOLS = [
-    "mgs_sort_temp_bytes", "mgs_sort", "mgs_merge_temp_bytes", "mgs_merge", "mgs_select_pivot",
+    "mgs_sort_temp_bytes", "mgs_sort", "mgs_sort_timed", "mgs_merge_temp_bytes", "mgs_merge", "mgs_select_pivot",
     "mgs_swap_ranges", "mgs_fill_max", "mgs_segmented_sort_temp_bytes", "mgs_segmented_sort",
     "mgs_status_str", "mgs_last_hip_error", "mgs_version",
 ]
@@ -47,6 +47,8 @@ def _load() -> ctypes.CDLL:
     lib.mgs_sort_temp_bytes.restype = sz
     lib.mgs_sort.argtypes = [i, i, vp, vp, vp, vp, sz, vp, sz, vp]
     lib.mgs_sort.restype = i
+    lib.mgs_sort_timed.argtypes = [i, i, vp, vp, vp, vp, sz, vp, sz, vp, ctypes.POINTER(vp), i, ctypes.POINTER(i)]
+    lib.mgs_sort_timed.restype = i
     lib.mgs_merge_temp_bytes.argtypes = [i, i, sz, sz]
     lib.mgs_merge_temp_bytes.restype = sz
     lib.mgs_merge.argtypes = [i, i, vp, vp, sz, vp, vp, sz, vp, vp, vp, sz, vp]

@@ -474,3 +474,20 @@ size_t oracle_nearly_sorted_racy_positions(size_t n, size_t* out) {
   }
   return c;
 }
+
+/* Known-answer hook: the k-th output (1-based) of mt19937 seeded with the
+ * standard integer seeding (C++ [rand.eng.mers]: x0 = s,
+ * xi = 1812433253 * (x(i-1) ^ (x(i-1) >> 30)) + i).  The standard fixes the
+ * 10000th output of seed 5489 at 4123659995 ([rand.predef]). */
+uint32_t oracle_mt19937_kat(uint32_t seed, int k) {
+  mt_t m;
+  m.x[0] = seed;
+  for (int i = 1; i < 624; ++i) m.x[i] = 1812433253u * (m.x[i - 1] ^ (m.x[i - 1] >> 30)) + (uint32_t)i;
+  m.p = 624;
+  uint32_t v = 0;
+  for (int i = 0; i < k; ++i) v = mt_next(&m);
+  return v;
+}
+
+/* Known-answer hook for seed_seq::generate: fills out[0, n) from v[0, s). */
+void oracle_seed_seq_generate(const uint32_t* v, size_t s, uint32_t* out, size_t n) { seed_seq_generate(v, s, out, n); }
