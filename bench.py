@@ -113,6 +113,28 @@ def cpu_baseline(H, np):
     return out
 
 
+def batched_section():
+    """4096 uneven arrays (log-uniform lengths in [2^8, 2^22]) of u64 keys with
+    u32 payloads: one graph-replayed mgs_segmented_sort vs one mgs_sort per
+    array (eager loop and the same loop captured in a graph)."""
+    from mgsort import batched
+    out = {"workload": "4096 arrays, lengths round(2^(8+14U)) (mt19937_64 seed 2024), u64 keys + u32 payload",
+           "bytes_per_key": 200}
+    for dist in ("uniform", "zipf-1.2"):
+        r = batched.run(dist=dist, log=log)
+        out[dist] = {"total_keys": r.total_keys, "batched_ms": round(r.batched_ms, 3),
+                     "loop_ms": round(r.loop_ms, 3),
+                     "loop_graph_ms": None if r.loop_graph_ms is None else round(r.loop_graph_ms, 3),
+                     "speedup_vs_loop": round(r.loop_ms / r.batched_ms, 3),
+                     "speedup_vs_loop_graph": None if r.loop_graph_ms is None else round(r.loop_graph_ms / r.batched_ms, 3),
+                     "batched_gkeys_s": round(r.total_keys / r.batched_ms / 1e6, 3),
+                     "batched_roofline_frac": round(r.total_keys * 200 / (r.batched_ms / 1e3) / (PEAK_GBPS * 1e9), 4),
+                     "tuning": r.tuning, "verified": r.verified}
+        if not r.verified:
+            raise RuntimeError(f"batched {dist}: output failed the check")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +142,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 30)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched segmented-sort section")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the sort path of this build runs on one GPU "
@@ -211,9 +234,12 @@ def main():
                      "alg_bytes_per_launch": n * PASS_BYTES_PER_KEY, "avg_launch_ms": round(avg_pass_ms, 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
     }
-    if not args.no_cpu:
-        del src, out, temp
+    del src, out, temp
+    torch.cuda.empty_cache()
+    if not args.no_batched:
+        result["batched"] = batched_section()
         torch.cuda.empty_cache()
+    if not args.no_cpu:
         result.update(cpu_baseline(H, np))
     print(json.dumps(result), flush=True)
 

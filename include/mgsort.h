@@ -156,6 +156,8 @@ int mgs_fill_max(mgs_key kt, void* keys, size_t n, void* stream);
  * nseg independent segments stably by enc(); segment s is
  * [offsets[s], offsets[s+1]) of keys/vals.  d_offsets is a DEVICE array of
  * nseg+1 uint64 offsets, non-decreasing, offsets[0] may be > 0.
+ * Offsets must not decrease (a segment starting before an earlier offset, or
+ * reaching past offsets[0] + max_total_keys, is left unsorted).
  * keys_in may equal keys_out.  `tuning` may be NULL.  One call enqueues a
  * fixed launch sequence independent of the segment lengths, so it can be
  * captured once in a hipGraph and replayed.  max_total_keys bounds
@@ -170,6 +172,23 @@ int mgs_segmented_sort(mgs_key kt, mgs_val vt,
                        const uint64_t* d_offsets, size_t nseg,
                        size_t max_total_keys, const mgs_tuning* tuning,
                        void* temp, size_t temp_bytes, void* stream);
+
+/* Ranges form of the segmented sort (no reference counterpart: the serving
+ * layer sorts requests that sit anywhere in its device pool).  Segment s is
+ * keys[d_ranges[2s], d_ranges[2s+1]), sorted in place with its payloads
+ * vals[v, v + len) where v = d_val_begin[s] (d_val_begin NULL: v =
+ * d_ranges[2s]); segments must not overlap and need not be adjacent or
+ * ordered.  d_ranges (2*nseg uint64) and d_val_begin (nseg uint64) are
+ * DEVICE arrays of element indices; the lengths may add up to at
+ * most max_total_keys (segments past that bound, or with end < begin, are left
+ * unsorted).  Scratch: mgs_segmented_sort_temp_bytes(kt, vt, max_total_keys,
+ * nseg).  Same fixed launch sequence as mgs_segmented_sort. */
+int mgs_segmented_sort_ranges(mgs_key kt, mgs_val vt,
+                              void* keys, void* vals,
+                              const uint64_t* d_ranges, const uint64_t* d_val_begin,
+                              size_t nseg,
+                              size_t max_total_keys, const mgs_tuning* tuning,
+                              void* temp, size_t temp_bytes, void* stream);
 
 /* Status text; never NULL. */
 const char* mgs_status_str(int status);

@@ -99,8 +99,20 @@ int mgs_segmented_sort(mgs_key kt, mgs_val vt, const void* keys_in, void* keys_o
   if (max_total_keys > kMaxKeys) return MGS_ERR_TOO_LARGE;
   if (keys_in == nullptr || keys_out == nullptr || d_offsets == nullptr) return MGS_ERR_INVALID;
   if (vt != MGS_VAL_NONE && (vals_in == nullptr || vals_out == nullptr)) return MGS_ERR_INVALID;
-  return mgs::segmented_dispatch(kt, vt, keys_in, keys_out, vals_in, vals_out, d_offsets, nseg, max_total_keys, tuning,
-                                 temp, temp_bytes, as_stream(stream));
+  return mgs::segmented_dispatch(kt, vt, keys_in, keys_out, vals_in, vals_out, d_offsets, false, nullptr, nseg, max_total_keys,
+                                 tuning, temp, temp_bytes, as_stream(stream));
+}
+
+int mgs_segmented_sort_ranges(mgs_key kt, mgs_val vt, void* keys, void* vals, const uint64_t* d_ranges,
+                              const uint64_t* d_val_begin, size_t nseg, size_t max_total_keys,
+                              const mgs_tuning* tuning, void* temp, size_t temp_bytes, void* stream) {
+  if (key_bytes(kt) < 0 || val_bytes(vt) < 0) return MGS_ERR_BAD_TYPE;
+  if (nseg == 0 || max_total_keys == 0) return MGS_OK;
+  if (max_total_keys > kMaxKeys) return MGS_ERR_TOO_LARGE;
+  if (keys == nullptr || d_ranges == nullptr) return MGS_ERR_INVALID;
+  if (vt != MGS_VAL_NONE && vals == nullptr) return MGS_ERR_INVALID;
+  return mgs::segmented_dispatch(kt, vt, keys, keys, vals, vals, d_ranges, true, d_val_begin, nseg, max_total_keys, tuning, temp,
+                                 temp_bytes, as_stream(stream));
 }
 
 const char* mgs_status_str(int status) {

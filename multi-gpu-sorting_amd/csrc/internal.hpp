@@ -49,8 +49,11 @@ int swap_dispatch(mgs_key kt, mgs_val vt, void* ak, void* av, void* bk, void* bv
 int fill_max_dispatch(mgs_key kt, void* keys, size_t n, hipStream_t s);
 
 size_t segmented_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total, size_t nseg);
+// ranges == false: d_bounds holds nseg+1 offsets; true: nseg (begin, end)
+// pairs, with payloads at d_val_begin[s] (NULL: at the keys' begin).
 int segmented_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout,
-                       const uint64_t* d_offsets, size_t nseg, size_t max_total, const mgs_tuning* tuning,
-                       void* temp, size_t temp_bytes, hipStream_t s);
+                       const uint64_t* d_bounds, bool ranges, const uint64_t* d_val_begin, size_t nseg,
+                       size_t max_total,
+                       const mgs_tuning* tuning, void* temp, size_t temp_bytes, hipStream_t s);
 
 }  // namespace mgs

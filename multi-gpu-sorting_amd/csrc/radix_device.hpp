@@ -10,6 +10,11 @@ namespace mgs {
 
 constexpr uint32_t kLookbackSpinLimit = 1u << 24;
 
+// Predecessor descriptors read per look-back round trip.
+#ifndef MGS_LB_WINDOW
+#define MGS_LB_WINDOW 8
+#endif
+
 // Wavefront ranking used by every kernel: one LDS atomic per item by
 // default; -DMGS_RANK_PEERS selects the peer-mask table instead.
 #ifdef MGS_RANK_PEERS
@@ -25,7 +30,7 @@ template <typename D>
 __device__ __forceinline__ uint32_t lookback(const D* desc, uint32_t tile, uint32_t d, int parity) {
   // Reads kWindow predecessors per round trip (independent loads in flight),
   // then consumes them in order.
-  constexpr int kWindow = 4;
+  constexpr int kWindow = MGS_LB_WINDOW;
   uint32_t prefix = 0;
   int64_t t = (int64_t)tile - 1;
   uint32_t spins = 0;

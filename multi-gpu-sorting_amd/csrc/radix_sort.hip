@@ -1,6 +1,9 @@
 // Onesweep LSD radix sort kernels and their launch sequence (see
 // radix_sort.hpp for the algorithm outline).
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "internal.hpp"
 #include "radix_device.hpp"
@@ -12,26 +15,33 @@ namespace mgs {
 // Pipeline (one call, fixed launch sequence, no host synchronisation):
 //
 //   init      zero histograms and tile counters, seed the OR/AND reductions
-//   upfront   one read: histogram of byte 0 of the encoded keys, bitwise OR
-//             and AND of all encoded keys (byte b is trivial -- one value for
-//             every key -- iff OR and AND agree on it); clears descriptors
+//   upfront   one read: per-chain histograms of byte 0 of the encoded keys,
+//             bitwise OR and AND of all encoded keys (byte b is trivial -- one
+//             value for every key -- iff OR and AND agree on it); clears the
+//             look-back descriptors
 //   pass p    p = 0..P-1, persistent workgroups; a trivial byte returns at
-//             once.  Each workgroup takes tiles from an atomic counter in
-//             order, prefetches the next tile's keys while it ranks the
-//             current one, looks back for its digit offsets, sorts the tile
-//             in LDS, writes digit runs with consecutive lanes on consecutive
-//             addresses and counts the digits of the next non-trivial byte on
-//             the way out (the next pass's histogram, flushed once per
-//             workgroup)
+//             once.  The pass's tiles form kChains chains of consecutive
+//             tiles; workgroup b serves chain b mod chains first (the XCD it
+//             runs on, so neighbouring tiles share one L2), then helps the
+//             others.  Per tile: rank the keys per wavefront (one LDS atomic
+//             each), publish the tile's digit counts, look back over the
+//             chain's predecessors for the digit prefix, sort the tile in LDS
+//             and write each digit run with consecutive lanes on consecutive
+//             addresses.  On the way out every key is counted in the
+//             histogram of the next non-trivial byte, per chain of its output
+//             position -- exactly the per-chain histograms the next pass
+//             starts from.
 //   fixup     copies the result into keys_out when the pass count left it in
 //             the scratch buffer
 // ---------------------------------------------------------------------------
 
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+
 __global__ void init_kernel(uint32_t* __restrict__ hist, uint32_t words, uint32_t* __restrict__ counters,
-                            uint64_t* __restrict__ red) {
+                            uint32_t counter_words, uint64_t* __restrict__ red) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t i = g; i < words; i += gridDim.x * blockDim.x) hist[i] = 0;
-  if (g < kMaxPasses) counters[g] = 0;
+  for (uint32_t i = g; i < counter_words; i += gridDim.x * blockDim.x) counters[i] = 0;
   if (g == 0) {
     red[0] = 0;     // OR identity
     red[1] = ~0ull;  // AND identity
@@ -51,26 +61,40 @@ __device__ __forceinline__ Bits wave_and(Bits v) {
   return v;
 }
 
-// Byte-0 histogram plus OR/AND of the encoded keys; also clears the look-back
-// descriptors (desc_words 32-bit words).
+// Chain of workgroup b in a chain-major split of `grid` workgroups, and the
+// workgroup's rank among those serving that chain.
+__device__ __forceinline__ void block_chain(uint32_t b, uint32_t grid, uint32_t chains, uint32_t& c, uint32_t& j,
+                                            uint32_t& nb) {
+  c = b % chains;
+  j = b / chains;
+  nb = (grid - c + chains - 1) / chains;
+}
+
+// Byte-0 histogram of one chain per workgroup plus OR/AND of the encoded
+// keys; also clears the look-back descriptors (desc_words 32-bit words).
+// Workgroups are split chain-major, so each flushes one 256-bin histogram.
 template <int KT, int BLOCK, bool VEC>
 __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>::Bits* __restrict__ keys, uint32_t n,
                                                         uint32_t* __restrict__ hist, uint64_t* __restrict__ red,
-                                                        uint32_t* __restrict__ desc, size_t desc_words) {
+                                                        uint32_t* __restrict__ desc, size_t desc_words, ChainMap cm) {
   using Bits = typename Codec<KT>::Bits;
   __shared__ uint32_t h[kRadix];
   __shared__ Bits s_or[BLOCK / kWave], s_and[BLOCK / kWave];
   const int tid = threadIdx.x;
   for (int i = tid; i < kRadix; i += BLOCK) h[i] = 0;
-  const size_t gtid = (size_t)blockIdx.x * BLOCK + tid;
-  const size_t gstride = (size_t)gridDim.x * BLOCK;
   {
+    const size_t gtid = (size_t)blockIdx.x * BLOCK + tid;
+    const size_t gstride = (size_t)gridDim.x * BLOCK;
     uint4* d4 = reinterpret_cast<uint4*>(desc);
     const size_t n4 = desc_words / 4;
     for (size_t i = gtid; i < n4; i += gstride) d4[i] = make_uint4(0, 0, 0, 0);
     for (size_t i = n4 * 4 + gtid; i < desc_words; i += gstride) desc[i] = 0;
   }
   __syncthreads();
+  uint32_t c, j, nb;
+  block_chain(blockIdx.x, gridDim.x, cm.chains, c, j, nb);
+  const size_t lo = (size_t)c * cm.chain_keys;
+  const size_t hi = std::min<size_t>((size_t)n, lo + cm.chain_keys);
   Bits vor = 0, vand = ~Bits(0);
   auto count = [&](Bits k) {
     const Bits e = Codec<KT>::enc(k);
@@ -79,11 +103,28 @@ __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>
     atomicAdd(&h[(uint32_t)e & 0xFFu], 1u);
   };
   if constexpr (VEC) {
+    // chain_keys is a multiple of the tile, so [lo, hi) starts 16-byte aligned
     constexpr int V = 16 / (int)sizeof(Bits);
+    constexpr int U = 4;  // vectors in flight per thread
     using Vec = typename std::conditional<sizeof(Bits) == 4, uint4, ulonglong2>::type;
-    const Vec* kv = reinterpret_cast<const Vec*>(keys);
-    const size_t nv = n / V;
-    for (size_t i = gtid; i < nv; i += gstride) {
+    const Vec* kv = reinterpret_cast<const Vec*>(keys + lo);
+    const size_t nv = (hi - lo) / V;
+    const size_t stride = (size_t)nb * BLOCK;
+    size_t i = (size_t)j * BLOCK + tid;
+    for (; i + (U - 1) * stride < nv; i += U * stride) {
+      Vec v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = kv[i + u * stride];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if constexpr (sizeof(Bits) == 4) {
+          count(v[u].x); count(v[u].y); count(v[u].z); count(v[u].w);
+        } else {
+          count(v[u].x); count(v[u].y);
+        }
+      }
+    }
+    for (; i < nv; i += stride) {
       const Vec v = kv[i];
       if constexpr (sizeof(Bits) == 4) {
         count(v.x); count(v.y); count(v.z); count(v.w);
@@ -91,9 +132,9 @@ __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>
         count(v.x); count(v.y);
       }
     }
-    for (size_t i = nv * V + gtid; i < n; i += gstride) count(keys[i]);
+    for (size_t k = lo + nv * V + (size_t)j * BLOCK + tid; k < hi; k += stride) count(keys[k]);
   } else {
-    for (size_t i = gtid; i < n; i += gstride) count(keys[i]);
+    for (size_t k = lo + (size_t)j * BLOCK + tid; k < hi; k += (size_t)nb * BLOCK) count(keys[k]);
   }
   vor = wave_or(vor);
   vand = wave_and(vand);
@@ -109,11 +150,12 @@ __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>
       b &= s_and[i];
     }
     atomicOr(reinterpret_cast<unsigned long long*>(&red[0]), (unsigned long long)a);
-    atomicAnd(reinterpret_cast<unsigned long long*>(&red[1]), (unsigned long long)b | (sizeof(Bits) == 4 ? 0xFFFFFFFF00000000ull : 0ull));
+    atomicAnd(reinterpret_cast<unsigned long long*>(&red[1]),
+              (unsigned long long)b | (sizeof(Bits) == 4 ? 0xFFFFFFFF00000000ull : 0ull));
   }
   for (int i = tid; i < kRadix; i += BLOCK) {
-    const uint32_t c = h[i];
-    if (c) atomicAdd(&hist[i], c);
+    const uint32_t v = h[i];
+    if (v) atomicAdd(&hist[c * kRadix + i], v);
   }
 }
 
@@ -143,17 +185,23 @@ __device__ __forceinline__ T* pick_dst(int id, T* alt, T* out) {
   return id == kBufAlt ? alt : out;
 }
 
-// Histogram of byte `b` over keys[0, n), added into hist (grid-stride).
+// Per-chain histogram of byte `b` of keys[0, n) (the input order), added
+// into hist_b[chain][256]; workgroups split chain-major.
 template <int KT, int BLOCK>
-__device__ void histogram_only(const typename Codec<KT>::Bits* keys, uint32_t n, int b, uint32_t* hist) {
+__device__ void histogram_only(const typename Codec<KT>::Bits* keys, uint32_t n, int b, uint32_t* hist_b,
+                               const ChainMap& cm) {
   __shared__ uint32_t h[kRadix];
   for (int i = threadIdx.x; i < kRadix; i += BLOCK) h[i] = 0;
   __syncthreads();
-  for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK)
+  uint32_t c, j, nb;
+  block_chain(blockIdx.x, gridDim.x, cm.chains, c, j, nb);
+  const size_t lo = (size_t)c * cm.chain_keys;
+  const size_t hi = std::min<size_t>((size_t)n, lo + cm.chain_keys);
+  for (size_t i = lo + (size_t)j * BLOCK + threadIdx.x; i < hi; i += (size_t)nb * BLOCK)
     atomicAdd(&h[digit_of<KT>(keys[i], 8 * b)], 1u);
   __syncthreads();
   for (int i = threadIdx.x; i < kRadix; i += BLOCK)
-    if (h[i]) atomicAdd(&hist[b * kRadix + i], h[i]);
+    if (h[i]) atomicAdd(&hist_b[c * kRadix + i], h[i]);
 }
 
 template <int KT, typename V, int IPT>
@@ -179,6 +227,21 @@ __device__ __forceinline__ void load_tile(const typename Codec<KT>::Bits* __rest
   }
 }
 
+// Next tile for a workgroup: tiles of its current chain in order, then of the
+// following chains once that one is used up.  Thread-private state: the
+// current chain and the number of chains not yet found empty.
+__device__ __forceinline__ uint32_t take_tile(uint32_t* __restrict__ cnt, const ChainMap& cm, uint32_t& c,
+                                              uint32_t& left) {
+  while (left > 0) {
+    const uint32_t t = atomicAdd(&cnt[c], 1u);
+    const uint32_t first = c * cm.tpc;
+    if (t < min(cm.tpc, cm.tiles - first)) return first + t;
+    c = (c + 1 == cm.chains) ? 0u : c + 1;
+    --left;
+  }
+  return kNoTile;
+}
+
 #ifdef MGS_PHASE_PROBE
 __device__ uint64_t* g_phase_probe;
 #define PHASE(slot)                                                                                 \
@@ -193,13 +256,14 @@ __device__ uint64_t* g_phase_probe;
 #endif
 
 // ---------------------------------------------------------------------------
-// One onesweep pass over byte `pass`, persistent workgroups.
+// One onesweep pass over byte `pass`, persistent workgroups, chained tiles.
+// hist is [P][kChains][256]; counters is [P][kChains].
 // ---------------------------------------------------------------------------
 template <int KT, typename V, int BLOCK, int IPT, typename D>
-__global__ __launch_bounds__(BLOCK) void pass_kernel(
+__global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
     const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
-    const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, uint32_t num_tiles,
-    const uint64_t* __restrict__ red, uint32_t* __restrict__ hist, uint32_t* __restrict__ tile_counter,
+    const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, ChainMap cm,
+    const uint64_t* __restrict__ red, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters,
     D* __restrict__ desc) {
   using Bits = typename Codec<KT>::Bits;
   constexpr int P = (int)sizeof(Bits);
@@ -215,7 +279,7 @@ __global__ __launch_bounds__(BLOCK) void pass_kernel(
     // one, nobody has counted that byte yet: do it here from the input.
     const uint32_t below = vary & ((2u << pass) - 1u);
     if (below == 0 && pass + 1 < P && (vary & (1u << (pass + 1))))
-      histogram_only<KT, BLOCK>(k_in, n, pass + 1, hist);
+      histogram_only<KT, BLOCK>(k_in, n, pass + 1, hist + (size_t)(pass + 1) * kChains * kRadix, cm);
     return;
   }
   const int k = __builtin_popcount(vary & ((1u << pass) - 1u));
@@ -232,71 +296,82 @@ __global__ __launch_bounds__(BLOCK) void pass_kernel(
     vdst = pick_dst(db, v_alt, v_out);
   }
   const int shift = 8 * pass, parity = k & 1;
+  uint32_t* __restrict__ cnt = counters + (size_t)pass * kChains;
+  const uint32_t* __restrict__ hp = hist + (size_t)pass * kChains * kRadix;
 
-  struct RankSmem {
-    uint64_t peers[W][kRadix];
-  };
-  union StageSmem {
-    RankSmem r;
-    Bits keys[TILE];
-  };
-  __shared__ StageSmem u;
+  __shared__ Bits keys[TILE];
   __shared__ VS vstage[HV ? TILE : 1];
+#ifdef MGS_RANK_PEERS
+  __shared__ uint64_t peers[W][kRadix];
+#endif
   __shared__ uint32_t off[W][kRadix];
+  __shared__ uint32_t nh[kChains][kRadix];
   __shared__ uint32_t gbase[kRadix];
-  __shared__ uint32_t base[kRadix];
-  __shared__ uint32_t nh[kRadix];
   __shared__ uint32_t sums[4];
   __shared__ uint32_t s_tile;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int w = tid / kWave;
+  uint32_t my_chain = blockIdx.x % cm.chains, chains_left = cm.chains;  // thread 0's acquisition state
 
-  {  // global digit bases of this byte
-    const uint32_t c = tid < kRadix ? hist[pass * kRadix + tid] : 0u;
-    const uint32_t e = block_excl_scan_256(c, sums, tid);
-    if (tid < kRadix) {
-      base[tid] = e;
-      nh[tid] = 0;
-    }
-    if (tid == 0) s_tile = atomicAdd(&tile_counter[pass], 1u);
+  // Digit base of this byte (keys with a smaller digit), held by the thread
+  // of each digit; the base of a chain adds the keys of earlier chains with
+  // the same digit, recomputed when the workgroup moves to another chain.
+  uint32_t dbase = 0, cur_chain = 0xFFFFFFFFu, cur_base = 0;
+  {
+    uint32_t tot = 0;
+    if (tid < kRadix)
+      for (uint32_t c = 0; c < cm.chains; ++c) tot += hp[c * kRadix + tid];
+    dbase = block_excl_scan_256(tot, sums, tid);
+    if (tid < kRadix)
+      for (uint32_t c = 0; c < (uint32_t)kChains; ++c) nh[c][tid] = 0;
+    for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
+#ifdef MGS_RANK_PEERS
+    for (int i = tid; i < W * kRadix; i += BLOCK) (&peers[0][0])[i] = 0;
+#endif
+    if (tid == 0) s_tile = take_tile(cnt, cm, my_chain, chains_left);
     __syncthreads();
   }
   uint32_t tile = s_tile;
   Bits kc[IPT];
   VS vc[IPT];
-  if (tile < num_tiles) {
+  if (tile != kNoTile) {
     const uint32_t tb = tile * (uint32_t)TILE;
     load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kc, vc);
   }
 
-  while (tile < num_tiles) {
-    PHASE(0);
-    for (int i = tid; i < W * kRadix; i += BLOCK) {
-      (&u.r.peers[0][0])[i] = 0;
-      (&off[0][0])[i] = 0;
-    }
-    __syncthreads();
+  while (tile != kNoTile) {
     PHASE(1);
-
+    const uint32_t chain = tile / cm.tpc;
+    const uint32_t first = chain * cm.tpc;
+    const uint32_t lt = tile - first;  // tile index inside its chain
     const uint32_t tile_base = tile * (uint32_t)TILE;
     const uint32_t tile_cnt = min((uint32_t)TILE, n - tile_base);
     const bool full = tile_cnt == (uint32_t)TILE;
     const uint32_t wb = tile_base + (uint32_t)(w * IPT * kWave + lane);
-    uint32_t dg[IPT], rk[IPT];
-#pragma unroll
-    for (int i = 0; i < IPT; ++i) dg[i] = digit_of<KT>(kc[i], shift);
-    uint64_t* pt = u.r.peers[w];
+    // per item: digit << 16 | rank among the wavefront's earlier items with
+    // that digit (< IPT * 64)
+    uint32_t pk[IPT];
     uint32_t* ct = off[w];
+#ifdef MGS_RANK_PEERS
+    uint64_t* pt = peers[w];
+#else
+    uint64_t* pt = nullptr;
+#endif
     (void)pt;
     if (full) {
 #pragma unroll
-      for (int i = 0; i < IPT; ++i) rk[i] = MGS_RANK(dg[i], pt, ct);
+      for (int i = 0; i < IPT; ++i) {
+        const uint32_t d = digit_of<KT>(kc[i], shift);
+        pk[i] = (d << 16) | MGS_RANK(d, pt, ct);
+      }
     } else {
 #pragma unroll
-      for (int i = 0; i < IPT; ++i)
-        if (wb + i * kWave < n) rk[i] = MGS_RANK(dg[i], pt, ct);
+      for (int i = 0; i < IPT; ++i) {
+        const uint32_t d = digit_of<KT>(kc[i], shift);
+        pk[i] = (wb + i * kWave < n) ? ((d << 16) | MGS_RANK(d, pt, ct)) : 0u;
+      }
     }
     __syncthreads();
     PHASE(2);
@@ -306,7 +381,7 @@ __global__ __launch_bounds__(BLOCK) void pass_kernel(
     if (tid < kRadix) {
 #pragma unroll
       for (int i = 0; i < W; ++i) total += off[i][tid];
-      st_agent(my_desc, Desc<D>::make(tile == 0 ? kStInclusive : kStAggregate, (D)total, parity));
+      st_agent(my_desc, Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)total, parity));
     }
     const uint32_t excl = block_excl_scan_256(total, sums, tid);
     if (tid < kRadix) {
@@ -318,11 +393,16 @@ __global__ __launch_bounds__(BLOCK) void pass_kernel(
         run += c;
       }
       uint32_t prefix = 0;
-      if (tile > 0) {
-        prefix = lookback(desc, tile, (uint32_t)tid, parity);
+      if (lt > 0) {
+        prefix = lookback(desc + (size_t)first * kRadix, lt, (uint32_t)tid, parity);
         st_agent(my_desc, Desc<D>::make(kStInclusive, (D)(prefix + total), parity));
       }
-      gbase[tid] = base[tid] + prefix - excl;
+      if (chain != cur_chain) {
+        cur_chain = chain;
+        cur_base = dbase;
+        for (uint32_t c = 0; c < chain; ++c) cur_base += hp[c * kRadix + tid];
+      }
+      gbase[tid] = cur_base + prefix - excl;
       if (tid == 0) PHASE(4);
     }
     __syncthreads();
@@ -331,38 +411,40 @@ __global__ __launch_bounds__(BLOCK) void pass_kernel(
     // tile, so a tile's predecessor is always one that is already being
     // ranked (taking it earlier makes look-backs wait on tiles nobody works
     // on yet).
-    if (tid == 0) s_tile = atomicAdd(&tile_counter[pass], 1u);
+    if (tid == 0) s_tile = take_tile(cnt, cm, my_chain, chains_left);
 
 #pragma unroll
     for (int i = 0; i < IPT; ++i) {
-      const uint32_t pos = off[w][dg[i]] + rk[i];
+      const uint32_t pos = off[w][pk[i] >> 16] + (pk[i] & 0xFFFFu);
       if ((full || wb + i * kWave < n) && pos < (uint32_t)TILE) {
-        u.keys[pos] = kc[i];
+        keys[pos] = kc[i];
         if constexpr (HV) vstage[pos] = vc[i];
       }
     }
     __syncthreads();
     const uint32_t nxt = s_tile;
 
-    // prefetch the next tile while this one is written out
+    // prefetch the next tile while this one is written out; the rank
+    // counters are free again and are cleared for it
     Bits kn[IPT];
     VS vn[IPT];
-    if (nxt < num_tiles) {
+    if (nxt != kNoTile) {
       const uint32_t tb = nxt * (uint32_t)TILE;
       load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kn, vn);
     }
+    for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
 
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
       const uint32_t i = (uint32_t)(j * BLOCK + tid);
       if (full || i < tile_cnt) {
-        const Bits kk = u.keys[i];
+        const Bits kk = keys[i];
         const uint32_t o = gbase[digit_of<KT>(kk, shift)] + i;
         if (o < n) {
           dst[o] = kk;
           if constexpr (HV) vdst[o] = vstage[i];
+          if (next >= 0) atomicAdd(&nh[chain_of(o, cm)][digit_of<KT>(kk, 8 * next)], 1u);
         }
-        if (next >= 0) atomicAdd(&nh[digit_of<KT>(kk, 8 * next)], 1u);
       }
     }
     PHASE(5);
@@ -375,7 +457,11 @@ __global__ __launch_bounds__(BLOCK) void pass_kernel(
       if constexpr (HV) vc[i] = vn[i];
     }
   }
-  if (next >= 0 && tid < kRadix && nh[tid]) atomicAdd(&hist[next * kRadix + tid], nh[tid]);
+  if (next >= 0 && tid < kRadix) {
+    uint32_t* hn = hist + (size_t)next * kChains * kRadix;
+    for (uint32_t c = 0; c < cm.chains; ++c)
+      if (nh[c][tid]) atomicAdd(&hn[c * kRadix + tid], nh[c][tid]);
+  }
 }
 
 // Copies the result into the output buffers when the passes left it in the
@@ -441,22 +527,36 @@ SortLayout sort_layout(size_t n) {
   };
   L.alt_k = take(n * KB);
   L.alt_v = take(n * VB);
-  L.hist = take((size_t)KB * kRadix * 4);
+  L.hist = take((size_t)KB * kChains * kRadix * 4);
   L.red = take(2 * sizeof(uint64_t));
-  L.counters = take(kMaxPasses * 4);
+  L.counters = take((size_t)KB * kChains * 4);
   L.desc = take((size_t)L.num_tiles * kRadix * (L.wide_desc ? 8 : 4));
   L.total = o;
   return L;
 }
 
-// Workgroups of `kernel` that fit on the device at once (persistent grid).
+// Workgroups of `kernel` that fit on the current device at once (persistent
+// grid).  Cached per (kernel, device), so that calls made while a stream is
+// being captured into a graph issue no device queries.
 template <typename K>
 uint32_t resident_blocks(K kernel, int block, size_t dyn_lds) {
-  int dev = 0, cus = 0, per_cu = 0;
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, uint32_t> cache;
+  int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
+  const auto key = std::make_pair(reinterpret_cast<const void*>(kernel), dev);
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  int cus = 0, per_cu = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, dyn_lds) != hipSuccess) return 0;
-  return (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
+  const uint32_t r = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
+  std::lock_guard<std::mutex> lock(mu);
+  cache[key] = r;
+  return r;
 }
 
 // Records events[i] (when given) before stage i; stage boundaries are the
@@ -506,22 +606,28 @@ int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t 
   const size_t desc_words = (size_t)L.num_tiles * kRadix * (L.wide_desc ? 2 : 1);
   const int aliased = (kin == kout) || (kHasVal<V> && vin == vout);
 
+  const ChainMap cm = make_chain_map(n, Cfg::kTile, kChains);
+
   mark(s);
-  init_kernel<<<1, 1024, 0, s>>>(hist, (uint32_t)(P * kRadix), counters, red);
+  init_kernel<<<1, 1024, 0, s>>>(hist, (uint32_t)(P * kChains * kRadix), counters, (uint32_t)(P * kChains), red);
   mark(s);
   {
     constexpr int HB = 256;
-    const unsigned grid = (unsigned)std::min<size_t>(std::max<size_t>((n + 16 * HB - 1) / (16 * HB), 1), 2048);
+    // chain-major split: the same number of workgroups per chain
+    const uint32_t per_chain = (uint32_t)std::min<size_t>(
+        std::max<size_t>((cm.chain_keys + 16 * HB - 1) / (16 * HB), 1), std::max<uint32_t>(2048 / cm.chains, 1));
+    const unsigned grid = per_chain * cm.chains;
     if ((reinterpret_cast<uintptr_t>(kin) % 16) == 0)
-      upfront_kernel<KT, HB, true><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words);
+      upfront_kernel<KT, HB, true><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words, cm);
     else
-      upfront_kernel<KT, HB, false><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words);
+      upfront_kernel<KT, HB, false><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words, cm);
   }
   auto launch_pass = [&](auto kernel, auto* dsc, int p) {
     const uint32_t res = resident_blocks(kernel, Cfg::kBlock, 0);
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(L.num_tiles, res ? res : L.num_tiles));
-    kernel<<<grid, Cfg::kBlock, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, p, aliased, L.num_tiles, red, hist,
-                                        counters, dsc);
+    uint32_t grid = std::min<uint32_t>(L.num_tiles, res ? res : L.num_tiles);
+    grid = std::max<uint32_t>(grid, cm.chains);  // every chain has a home workgroup
+    kernel<<<grid, Cfg::kBlock, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, p, aliased, cm, red, hist, counters,
+                                        dsc);
   };
   for (int p = 0; p < P; ++p) {
     mark(s);

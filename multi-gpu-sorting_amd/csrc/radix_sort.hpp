@@ -1,31 +1,17 @@
-// LSD onesweep radix sort for gfx950 (MI355X).
+// LSD onesweep radix sort for gfx950 (MI355X): shared definitions.
 //
 // Replaces the thrust::sort / cub::DeviceRadixSort call sites of the
 // reference (src/tanasic_sort.cuh:264-266, src/gowanlock_sort.cuh:188-190,
 // :337-340, src/sort_benchmark.cu:89, src/local_gpu_sort_algorithm_benchmark.cu
 // :55-65).  Algorithm: Adinets & Merrill, "Onesweep" (2022) with the
 // decoupled look-back of Merrill & Garland (2016), re-designed for 64-lane
-// wavefronts:
-//
-//   init      zero the digit histograms and tile counters
-//   hist      one read of all keys -> the 256-bin histogram of every byte;
-//             also clears the look-back descriptors
-//   plan      exclusive scan of each histogram (global digit bases), marks
-//             bytes whose histogram has a single bin as trivial (identity
-//             permutation, skipped) and assigns ping-pong buffers so that the
-//             result lands in keys_out and the input is never written unless
-//             keys_in == keys_out
-//   pass p    (one launch per byte) each workgroup takes the next tile from an
-//             atomic counter, ranks its keys per wavefront with an LDS peer
-//             table (stable, no reliance on atomic ordering), publishes its
-//             digit counts, looks back over predecessor tiles for the global
-//             prefix, sorts the tile in LDS and writes each digit run with
-//             consecutive lanes on consecutive addresses
-//   fixup     copies the result into keys_out when the number of non-trivial
-//             passes left it in the scratch buffer
-//
-// Small inputs (one tile) are sorted by a single workgroup entirely in LDS.
+// wavefronts and the eight L2 domains (XCDs) of MI355X: every pass splits
+// its tiles into independent look-back chains, one per XCD, whose digit bases
+// come from per-chain histograms counted by the previous pass.  The launch
+// sequence is described in radix_sort.hip.
 #pragma once
+
+#include <algorithm>
 
 #include "common.hpp"
 
@@ -33,17 +19,47 @@ namespace mgs {
 
 constexpr int kMaxPasses = 8;
 
-// Buffer ids used by the pass plan.
+// Buffer ids of the ping-pong plan.
 enum : int32_t { kBufIn = 0, kBufAlt = 1, kBufOut = 2 };
 
-struct PassPlan {
-  int32_t num_active;
-  int32_t k_of_pass[kMaxPasses];  // active index of pass p, -1 if trivial
-  int32_t src[kMaxPasses];        // indexed by active index
-  int32_t dst[kMaxPasses];
-  int32_t final_copy_src;         // -1: none
-  int32_t pad[3];
+// Look-back chains per pass (one per XCD by default).
+#ifndef MGS_CHAINS
+#define MGS_CHAINS 8
+#endif
+constexpr int kChains = MGS_CHAINS;
+
+// Partition of a pass's tiles into chains: chain c holds the tiles
+// [c*tpc, min((c+1)*tpc, tiles)), i.e. the keys [c*chain_keys, ...).
+// chain_of() maps a key position to its chain with one multiply by a
+// precomputed reciprocal (magic = ceil(2^40 / (chain_keys >> shift))) and one
+// correction step, exact for every position below 2^32.
+struct ChainMap {
+  uint32_t tiles;
+  uint32_t tpc;
+  uint32_t chains;
+  uint32_t shift;
+  uint64_t chain_keys;
+  uint64_t magic;
 };
+
+__host__ __device__ inline uint32_t chain_of(uint32_t pos, const ChainMap& m) {
+  uint32_t c = (uint32_t)(((uint64_t)(pos >> m.shift) * m.magic) >> 40);
+  if ((uint64_t)c * m.chain_keys > pos) --c;
+  return c;
+}
+
+inline ChainMap make_chain_map(size_t n, uint32_t tile, int max_chains) {
+  ChainMap m{};
+  m.tiles = (uint32_t)((n + tile - 1) / tile);
+  uint32_t ch = (uint32_t)std::max(1, std::min<int>(max_chains, (int)std::max<uint32_t>(m.tiles, 1u)));
+  m.tpc = std::max<uint32_t>(1u, (m.tiles + ch - 1) / ch);
+  m.chains = std::max<uint32_t>(1u, (m.tiles + m.tpc - 1) / m.tpc);
+  m.shift = (uint32_t)__builtin_ctz(tile);
+  m.chain_keys = (uint64_t)m.tpc * tile;
+  const uint64_t d = m.chain_keys >> m.shift;
+  m.magic = ((uint64_t(1) << 40) + d - 1) / d;
+  return m;
+}
 
 // Look-back descriptor: 2-bit status code in the top bits, count below.
 // The status code is rotated by the parity of the active pass index so that
@@ -73,9 +89,11 @@ enum : uint32_t { kStEmpty = 0, kStAggregate = 1, kStInclusive = 2 };
 #ifndef MGS_OS_IPT
 #define MGS_OS_IPT 16
 #endif
+// A tile stages at most 48 KiB of keys and payloads in LDS, so that two
+// workgroups fit on one CU beside their counters.
 template <int KB, int VB> struct TileCfg {
   static constexpr int kBlock = MGS_OS_BLOCK;
-  static constexpr int kIpt = (KB + VB) <= 8 ? MGS_OS_IPT : MGS_OS_IPT / 2;
+  static constexpr int kIpt = (KB + VB) <= 4 ? MGS_OS_IPT : ((KB + VB) <= 12 ? MGS_OS_IPT / 2 : MGS_OS_IPT / 4);
   static constexpr int kTile = kBlock * kIpt;
 };
 

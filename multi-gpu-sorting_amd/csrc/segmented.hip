@@ -2,7 +2,10 @@
 // sequence whose grid sizes depend only on (nseg, max_total_keys), so a call
 // can be captured once in a hipGraph and replayed for new data.
 //
-//   classify  one workgroup: bins each segment by length
+//   classify  one workgroup: resolves each segment's position (offsets form:
+//             [offsets[s], offsets[s+1]); ranges form: [ranges[2s],
+//             ranges[2s+1]) anywhere in the array, non-overlapping) and its
+//             place in the compact scratch buffer, then bins it by length
 //               small  (len <= small_max)  -> one workgroup sorts it in LDS
 //               medium (len <= medium_max) -> one workgroup streams it through
 //                                             HBM pass by pass, no look-back
@@ -27,6 +30,17 @@ namespace {
 
 struct SegCtl {
   uint32_t n_small, n_medium, n_large, large_tiles;
+  uint32_t skipped;  // segments left unsorted: malformed bounds or beyond max_total
+};
+
+// Per-segment table written by classify: first key in the key array, first
+// payload in the payload array, first slot in the compact scratch buffer,
+// length.
+struct SegTab {
+  uint64_t* begin;
+  uint64_t* vbegin;
+  uint64_t* alt;
+  uint32_t* len;
 };
 
 constexpr int kClassifyBlock = 1024;
@@ -62,30 +76,113 @@ __device__ inline void block_scan3(uint32_t a, uint32_t b, uint32_t c, uint32_t&
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kClassifyBlock) void seg_classify_kernel(const uint64_t* __restrict__ offsets,
-                                                                      uint32_t nseg, uint32_t small_max,
-                                                                      uint32_t medium_max, uint32_t tile,
+// Exclusive scan of 64-bit values across a 1024-thread block; *total gets
+// the block sum.
+__device__ inline uint64_t block_excl_scan_u64(uint64_t v, uint64_t* sm, uint64_t* total) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint64_t incl = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint64_t t = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) sm[w] = incl;
+  __syncthreads();
+  uint64_t base = 0, sum = 0;
+  for (int i = 0; i < kClassifyBlock / 64; ++i) {
+    if (i < w) base += sm[i];
+    sum += sm[i];
+  }
+  *total = sum;
+  __syncthreads();
+  return base + incl - v;
+}
+
+// Inclusive running maximum of 64-bit values across a 1024-thread block;
+// *total gets the block maximum.
+__device__ inline uint64_t block_incl_max_u64(uint64_t v, uint64_t* sm, uint64_t* total) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint64_t incl = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint64_t t = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl = incl > t ? incl : t;
+  }
+  if (lane == 63) sm[w] = incl;
+  __syncthreads();
+  uint64_t base = 0, all = 0;
+  for (int i = 0; i < kClassifyBlock / 64; ++i) {
+    if (i < w) base = base > sm[i] ? base : sm[i];
+    all = all > sm[i] ? all : sm[i];
+  }
+  *total = all;
+  __syncthreads();
+  return incl > base ? incl : base;
+}
+
+template <bool RANGES>
+__global__ __launch_bounds__(kClassifyBlock) void seg_classify_kernel(const uint64_t* __restrict__ bounds,
+                                                                      const uint64_t* __restrict__ val_begin,
+                                                                      uint32_t nseg, uint64_t max_total,
+                                                                      uint32_t small_max, uint32_t medium_max,
+                                                                      uint32_t tile, SegTab tab,
                                                                       uint32_t* small_list, uint32_t* medium_list,
                                                                       uint32_t* large_list, uint32_t* tile_start,
                                                                       SegCtl* ctl, uint32_t* tile_counter) {
   __shared__ uint32_t sm[3][16];
-  __shared__ uint32_t sm_t[16];
+  __shared__ uint64_t sm64[16];
   const int tid = threadIdx.x;
   if (tid < kMaxPasses + 1) tile_counter[tid] = 0;
-  uint32_t run_s = 0, run_m = 0, run_l = 0, run_t = 0;
+  uint32_t run_s = 0, run_m = 0, run_l = 0, run_t = 0, skipped = 0;
+  uint64_t run_alt = 0, run_max = 0;
+  const uint64_t base0 = RANGES ? 0 : bounds[0];
   for (uint32_t base = 0; base < nseg; base += kClassifyBlock) {
     const uint32_t s = base + tid;
-    uint32_t len = 0;
-    if (s < nseg) len = (uint32_t)(offsets[s + 1] - offsets[s]);
+    uint64_t b = 0, want = 0;
+    bool ok = true;
+    if (s < nseg) {
+      const uint64_t lo = RANGES ? bounds[2 * (size_t)s] : bounds[s];
+      const uint64_t hi = RANGES ? bounds[2 * (size_t)s + 1] : bounds[s + 1];
+      b = lo;
+      ok = hi >= lo && hi - lo < (uint64_t(1) << 32);
+      want = ok ? hi - lo : 0;
+    }
+    // compact scratch slot: the offsets form keeps the segment's distance
+    // from the first offset; the ranges form packs the segments in order
+    uint64_t alt = 0, chunk = 0;
+    if (RANGES) {
+      alt = run_alt + block_excl_scan_u64(want, sm64, &chunk);
+      run_alt += chunk;
+    } else {
+      // offsets must not decrease: a segment is taken only if it starts at
+      // or after every earlier offset, so the taken segments are disjoint
+      // (b is at least every earlier offset iff the running maximum up to
+      // and including it is b itself)
+      const uint64_t m = block_incl_max_u64(s < nseg ? b : 0, sm64, &chunk);
+      if (s < nseg && (m != b || b < run_max)) ok = false;
+      run_max = run_max > chunk ? run_max : chunk;
+      alt = b - base0;
+    }
+    if (s < nseg && (!ok || (!RANGES && b < base0) || alt > max_total || want > max_total - alt)) {
+      ok = false;  // left unsorted rather than running past the scratch buffer
+      want = 0;
+    }
+    const uint32_t len = (uint32_t)want;
+    if (s < nseg) {
+      tab.begin[s] = b;
+      tab.vbegin[s] = (RANGES && val_begin != nullptr) ? val_begin[s] : b;
+      tab.alt[s] = alt;
+      tab.len[s] = len;
+    }
+    const uint32_t bad = (s < nseg && !ok) ? 1u : 0u;
     const uint32_t is_s = (s < nseg && len > 0 && len <= small_max) ? 1u : 0u;
     const uint32_t is_m = (s < nseg && len > small_max && len <= medium_max) ? 1u : 0u;
     const uint32_t is_l = (s < nseg && len > medium_max) ? 1u : 0u;
     const uint32_t nt = is_l ? (len + tile - 1) / tile : 0u;
     uint32_t es, em, el, ts, tm, tl;
     block_scan3(is_s, is_m, is_l, es, em, el, ts, tm, tl, sm);
-    // tiles scan (separately, reusing the 3-way scan with zeros)
-    uint32_t et, d0, d1, tt, t0, t1;
-    block_scan3(nt, 0, 0, et, d0, d1, tt, t0, t1, sm);
+    uint32_t et, eb, e2, tt, tb, t2;
+    block_scan3(nt, bad, 0, et, eb, e2, tt, tb, t2, sm);
     if (is_s) small_list[run_s + es] = s;
     if (is_m) medium_list[run_m + em] = s;
     if (is_l) {
@@ -96,15 +193,16 @@ __global__ __launch_bounds__(kClassifyBlock) void seg_classify_kernel(const uint
     run_m += tm;
     run_l += tl;
     run_t += tt;
+    skipped += tb;
   }
   if (tid == 0) {
     ctl->n_small = run_s;
     ctl->n_medium = run_m;
     ctl->n_large = run_l;
     ctl->large_tiles = run_t;
+    ctl->skipped = skipped;
     tile_start[run_l] = run_t;
   }
-  (void)sm_t;
 }
 
 __global__ void seg_clear_kernel(uint32_t* __restrict__ hist, uint32_t hist_words_per_seg,
@@ -121,19 +219,18 @@ __global__ void seg_clear_kernel(uint32_t* __restrict__ hist, uint32_t hist_word
 template <int KT, typename V, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void seg_small_kernel(const typename Codec<KT>::Bits* k_in,
                                                           typename Codec<KT>::Bits* k_out, const V* v_in, V* v_out,
-                                                          const uint64_t* __restrict__ offsets,
-                                                          const uint32_t* __restrict__ list,
+                                                          SegTab tab, const uint32_t* __restrict__ list,
                                                           const SegCtl* __restrict__ ctl) {
   const uint32_t cnt = ctl->n_small;
   for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
     const uint32_t s = list[i];
-    const uint64_t b = offsets[s];
-    const uint32_t len = (uint32_t)(offsets[s + 1] - b);
+    const uint64_t b = tab.begin[s];
+    const uint32_t len = tab.len[s];
     const V* vi = nullptr;
     V* vo = nullptr;
     if constexpr (kHasVal<V>) {
-      vi = v_in + b;
-      vo = v_out + b;
+      vi = v_in + tab.vbegin[s];
+      vo = v_out + tab.vbegin[s];
     }
     small_sort_block<KT, V, BLOCK, IPT>(k_in + b, k_out + b, vi, vo, len);
     __syncthreads();
@@ -270,22 +367,21 @@ template <int KT, typename V, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void seg_medium_kernel(const typename Codec<KT>::Bits* k_in,
                                                            typename Codec<KT>::Bits* k_alt,
                                                            typename Codec<KT>::Bits* k_out, const V* v_in, V* v_alt,
-                                                           V* v_out, const uint64_t* __restrict__ offsets,
-                                                           const uint32_t* __restrict__ list,
+                                                           V* v_out, SegTab tab, const uint32_t* __restrict__ list,
                                                            const SegCtl* __restrict__ ctl) {
   const uint32_t cnt = ctl->n_medium;
-  const uint64_t base0 = offsets[0];
   for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
     const uint32_t s = list[i];
-    const uint64_t b = offsets[s];
-    const uint32_t len = (uint32_t)(offsets[s + 1] - b);
-    const uint64_t ab = b - base0;  // scratch is indexed from offsets[0]
+    const uint64_t b = tab.begin[s];
+    const uint32_t len = tab.len[s];
+    const uint64_t ab = tab.alt[s];
     const V* vi = nullptr;
     V *va = nullptr, *vo = nullptr;
     if constexpr (kHasVal<V>) {
-      vi = v_in + b;
+      const uint64_t vb = tab.vbegin[s];
+      vi = v_in + vb;
       va = v_alt + ab;
-      vo = v_out + b;
+      vo = v_out + vb;
     }
     medium_sort_block<KT, V, BLOCK, IPT>(k_in + b, k_alt + ab, k_out + b, vi, va, vo, len);
     __syncthreads();
@@ -307,8 +403,7 @@ __device__ __forceinline__ uint32_t find_large(const uint32_t* tile_start, uint3
 
 template <int KT, int BLOCK, int TILE>
 __global__ __launch_bounds__(BLOCK) void seg_hist_kernel(const typename Codec<KT>::Bits* __restrict__ k_in,
-                                                         const uint64_t* __restrict__ offsets,
-                                                         const uint32_t* __restrict__ large_list,
+                                                         SegTab tab, const uint32_t* __restrict__ large_list,
                                                          const uint32_t* __restrict__ tile_start,
                                                          uint32_t* __restrict__ seg_hist,
                                                          const SegCtl* __restrict__ ctl) {
@@ -322,8 +417,8 @@ __global__ __launch_bounds__(BLOCK) void seg_hist_kernel(const typename Codec<KT
   __syncthreads();
   const uint32_t li = find_large(tile_start, ctl->n_large, g);
   const uint32_t s = large_list[li];
-  const uint64_t b = offsets[s];
-  const uint32_t len = (uint32_t)(offsets[s + 1] - b);
+  const uint64_t b = tab.begin[s];
+  const uint32_t len = tab.len[s];
   const uint32_t t0 = (g - tile_start[li]) * (uint32_t)TILE;
   const uint32_t t1 = min(len, t0 + (uint32_t)TILE);
   for (uint32_t i = t0 + tid; i < t1; i += BLOCK) {
@@ -359,7 +454,7 @@ template <int KT, typename V, int BLOCK, int IPT, typename D>
 __global__ __launch_bounds__(BLOCK) void seg_pass_kernel(const typename Codec<KT>::Bits* k_in,
                                                          typename Codec<KT>::Bits* k_alt,
                                                          typename Codec<KT>::Bits* k_out, const V* v_in, V* v_alt,
-                                                         V* v_out, const uint64_t* __restrict__ offsets,
+                                                         V* v_out, SegTab tab,
                                                          const uint32_t* __restrict__ large_list,
                                                          const uint32_t* __restrict__ tile_start,
                                                          const uint32_t* __restrict__ seg_base,
@@ -374,9 +469,9 @@ __global__ __launch_bounds__(BLOCK) void seg_pass_kernel(const typename Codec<KT
   if (g >= ctl->large_tiles) return;
   const uint32_t li = find_large(tile_start, ctl->n_large, g);
   const uint32_t s = large_list[li];
-  const uint64_t b = offsets[s];
-  const uint64_t ab = b - offsets[0];
-  const uint32_t len = (uint32_t)(offsets[s + 1] - b);
+  const uint64_t b = tab.begin[s];
+  const uint64_t ab = tab.alt[s];
+  const uint32_t len = tab.len[s];
   const Bits* src;
   Bits* dst;
   if (pass == 0) {
@@ -392,14 +487,15 @@ __global__ __launch_bounds__(BLOCK) void seg_pass_kernel(const typename Codec<KT
   const V* vsrc = nullptr;
   V* vdst = nullptr;
   if constexpr (kHasVal<V>) {
+    const uint64_t vb = tab.vbegin[s];
     if (pass == 0) {
-      vsrc = v_in + b;
+      vsrc = v_in + vb;
       vdst = v_alt + ab;
     } else if (pass & 1) {
       vsrc = v_alt + ab;
-      vdst = v_out + b;
+      vdst = v_out + vb;
     } else {
-      vsrc = v_out + b;
+      vsrc = v_out + vb;
       vdst = v_alt + ab;
     }
   }
@@ -412,8 +508,8 @@ constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 struct SegLayout {
-  size_t alt_k, alt_v, ctl, small_list, medium_list, large_list, tile_start, seg_hist, seg_base, tile_counter, desc,
-      total;
+  size_t alt_k, alt_v, ctl, tab_begin, tab_vbegin, tab_alt, tab_len, small_list, medium_list, large_list, tile_start, seg_hist,
+      seg_base, tile_counter, desc, total;
   uint32_t max_large, tiles_ub;
   bool wide_desc;
 };
@@ -445,6 +541,14 @@ SegLayout seg_layout(size_t max_total, size_t nseg) {
   o = align_up(o + max_total * VB);
   L.ctl = o;
   o = align_up(o + sizeof(SegCtl));
+  L.tab_begin = o;
+  o = align_up(o + nseg * 8);
+  L.tab_vbegin = o;
+  o = align_up(o + nseg * 8);
+  L.tab_alt = o;
+  o = align_up(o + nseg * 8);
+  L.tab_len = o;
+  o = align_up(o + nseg * 4);
   L.small_list = o;
   o = align_up(o + nseg * 4);
   L.medium_list = o;
@@ -465,8 +569,8 @@ SegLayout seg_layout(size_t max_total, size_t nseg) {
 }
 
 template <int KT, int VT>
-int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const uint64_t* offsets, size_t nseg,
-              size_t max_total, const mgs_tuning* tuning, void* temp, size_t temp_bytes, hipStream_t s) {
+int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const uint64_t* bounds, bool ranges,
+              const uint64_t* val_begin, size_t nseg, size_t max_total, const mgs_tuning* tuning, void* temp, size_t temp_bytes, hipStream_t s) {
   using Bits = typename Codec<KT>::Bits;
   using V = typename ValOf<VT>::T;
   constexpr int KB = sizeof(Bits);
@@ -497,44 +601,56 @@ int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const ui
   uint32_t* seg_base = reinterpret_cast<uint32_t*>(t + L.seg_base);
   uint32_t* tcount = reinterpret_cast<uint32_t*>(t + L.tile_counter);
   void* desc = t + L.desc;
+  const SegTab tab{reinterpret_cast<uint64_t*>(t + L.tab_begin), reinterpret_cast<uint64_t*>(t + L.tab_vbegin),
+                   reinterpret_cast<uint64_t*>(t + L.tab_alt), reinterpret_cast<uint32_t*>(t + L.tab_len)};
 
   const Bits* ki = static_cast<const Bits*>(kin);
   Bits* ko = static_cast<Bits*>(kout);
   const V* vi = static_cast<const V*>(vin);
   V* vo = static_cast<V*>(vout);
 
-  seg_classify_kernel<<<1, kClassifyBlock, 0, s>>>(offsets, (uint32_t)nseg, small_max, medium_max, TC::kTile,
-                                                   small_list, medium_list, large_list, tile_start, ctl, tcount);
+  if (ranges)
+    seg_classify_kernel<true><<<1, kClassifyBlock, 0, s>>>(bounds, val_begin, (uint32_t)nseg, max_total, small_max, medium_max,
+                                                           TC::kTile, tab, small_list, medium_list, large_list,
+                                                           tile_start, ctl, tcount);
+  else
+    seg_classify_kernel<false><<<1, kClassifyBlock, 0, s>>>(bounds, val_begin, (uint32_t)nseg, max_total, small_max, medium_max,
+                                                            TC::kTile, tab, small_list, medium_list, large_list,
+                                                            tile_start, ctl, tcount);
   seg_clear_kernel<<<2048, 256, 0, s>>>(seg_hist, P * kRadix, (uint32_t*)desc, kRadix * (L.wide_desc ? 2 : 1), ctl);
   {
     const unsigned grid = (unsigned)std::min<size_t>(nseg, 8192);
-    seg_small_kernel<KT, V, SC::kBlock, SC::kIpt><<<grid, SC::kBlock, 0, s>>>(ki, ko, vi, vo, offsets, small_list, ctl);
+    seg_small_kernel<KT, V, SC::kBlock, SC::kIpt><<<grid, SC::kBlock, 0, s>>>(ki, ko, vi, vo, tab, small_list, ctl);
     seg_medium_kernel<KT, V, TC::kBlock, TC::kIpt>
-        <<<grid, TC::kBlock, 0, s>>>(ki, ka, ko, vi, (V*)va, vo, offsets, medium_list, ctl);
+        <<<grid, TC::kBlock, 0, s>>>(ki, ka, ko, vi, (V*)va, vo, tab, medium_list, ctl);
   }
-  seg_hist_kernel<KT, 256, TC::kTile><<<L.tiles_ub, 256, 0, s>>>(ki, offsets, large_list, tile_start, seg_hist, ctl);
+  seg_hist_kernel<KT, 256, TC::kTile><<<L.tiles_ub, 256, 0, s>>>(ki, tab, large_list, tile_start, seg_hist, ctl);
   seg_scan_kernel<P><<<L.max_large, 256, 0, s>>>(seg_hist, seg_base, ctl);
   for (int p = 0; p < P; ++p) {
     if (L.wide_desc)
       seg_pass_kernel<KT, V, TC::kBlock, TC::kIpt, uint64_t><<<L.tiles_ub, TC::kBlock, 0, s>>>(
-          ki, ka, ko, vi, (V*)va, vo, offsets, large_list, tile_start, seg_base, tcount, (uint64_t*)desc, ctl, p);
+          ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_base, tcount, (uint64_t*)desc, ctl, p);
     else
       seg_pass_kernel<KT, V, TC::kBlock, TC::kIpt, uint32_t><<<L.tiles_ub, TC::kBlock, 0, s>>>(
-          ki, ka, ko, vi, (V*)va, vo, offsets, large_list, tile_start, seg_base, tcount, (uint32_t*)desc, ctl, p);
+          ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_base, tcount, (uint32_t*)desc, ctl, p);
   }
   return check_launch();
 }
 
 template <int KT>
 int seg_by_val(mgs_val vt, const void* kin, void* kout, const void* vin, void* vout, const uint64_t* offsets,
-               size_t nseg, size_t max_total, const mgs_tuning* tuning, void* temp, size_t temp_bytes, hipStream_t s) {
+               bool ranges, const uint64_t* val_begin, size_t nseg, size_t max_total, const mgs_tuning* tuning, void* temp, size_t temp_bytes,
+               hipStream_t s) {
   switch (vt) {
     case MGS_VAL_NONE:
-      return seg_typed<KT, MGS_VAL_NONE>(kin, kout, vin, vout, offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+      return seg_typed<KT, MGS_VAL_NONE>(kin, kout, vin, vout, offsets, ranges, val_begin, nseg, max_total, tuning, temp,
+                                                 temp_bytes, s);
     case MGS_VAL_U32:
-      return seg_typed<KT, MGS_VAL_U32>(kin, kout, vin, vout, offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+      return seg_typed<KT, MGS_VAL_U32>(kin, kout, vin, vout, offsets, ranges, val_begin, nseg, max_total, tuning, temp,
+                                                 temp_bytes, s);
     case MGS_VAL_U64:
-      return seg_typed<KT, MGS_VAL_U64>(kin, kout, vin, vout, offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+      return seg_typed<KT, MGS_VAL_U64>(kin, kout, vin, vout, offsets, ranges, val_begin, nseg, max_total, tuning, temp,
+                                                 temp_bytes, s);
   }
   return MGS_ERR_BAD_TYPE;
 }
@@ -560,21 +676,21 @@ size_t segmented_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total, size_t nse
 }
 
 int segmented_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout,
-                       const uint64_t* d_offsets, size_t nseg, size_t max_total, const mgs_tuning* tuning,
+                       const uint64_t* d_offsets, bool ranges, const uint64_t* d_val_begin, size_t nseg, size_t max_total, const mgs_tuning* tuning,
                        void* temp, size_t temp_bytes, hipStream_t s) {
   switch (kt) {
     case MGS_KEY_U32:
-      return seg_by_val<MGS_KEY_U32>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+      return seg_by_val<MGS_KEY_U32>(vt, kin, kout, vin, vout, d_offsets, ranges, d_val_begin, nseg, max_total, tuning, temp, temp_bytes, s);
     case MGS_KEY_U64:
-      return seg_by_val<MGS_KEY_U64>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+      return seg_by_val<MGS_KEY_U64>(vt, kin, kout, vin, vout, d_offsets, ranges, d_val_begin, nseg, max_total, tuning, temp, temp_bytes, s);
     case MGS_KEY_I32:
-      return seg_by_val<MGS_KEY_I32>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+      return seg_by_val<MGS_KEY_I32>(vt, kin, kout, vin, vout, d_offsets, ranges, d_val_begin, nseg, max_total, tuning, temp, temp_bytes, s);
     case MGS_KEY_I64:
-      return seg_by_val<MGS_KEY_I64>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+      return seg_by_val<MGS_KEY_I64>(vt, kin, kout, vin, vout, d_offsets, ranges, d_val_begin, nseg, max_total, tuning, temp, temp_bytes, s);
     case MGS_KEY_F32:
-      return seg_by_val<MGS_KEY_F32>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+      return seg_by_val<MGS_KEY_F32>(vt, kin, kout, vin, vout, d_offsets, ranges, d_val_begin, nseg, max_total, tuning, temp, temp_bytes, s);
     case MGS_KEY_F64:
-      return seg_by_val<MGS_KEY_F64>(vt, kin, kout, vin, vout, d_offsets, nseg, max_total, tuning, temp, temp_bytes, s);
+      return seg_by_val<MGS_KEY_F64>(vt, kin, kout, vin, vout, d_offsets, ranges, d_val_begin, nseg, max_total, tuning, temp, temp_bytes, s);
   }
   return MGS_ERR_BAD_TYPE;
 }

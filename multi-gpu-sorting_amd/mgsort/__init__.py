@@ -29,7 +29,7 @@ MGS_OK = 0
 
 SYMBOLS = [
     "mgs_sort_temp_bytes", "mgs_sort", "mgs_sort_timed", "mgs_merge_temp_bytes", "mgs_merge", "mgs_select_pivot",
-    "mgs_swap_ranges", "mgs_fill_max", "mgs_segmented_sort_temp_bytes", "mgs_segmented_sort",
+    "mgs_swap_ranges", "mgs_fill_max", "mgs_segmented_sort_temp_bytes", "mgs_segmented_sort", "mgs_segmented_sort_ranges",
     "mgs_status_str", "mgs_last_hip_error", "mgs_version",
 ]
 
@@ -63,6 +63,8 @@ def _load() -> ctypes.CDLL:
     lib.mgs_segmented_sort_temp_bytes.restype = sz
     lib.mgs_segmented_sort.argtypes = [i, i, vp, vp, vp, vp, vp, sz, sz, vp, vp, sz, vp]
     lib.mgs_segmented_sort.restype = i
+    lib.mgs_segmented_sort_ranges.argtypes = [i, i, vp, vp, vp, vp, sz, sz, vp, vp, sz, vp]
+    lib.mgs_segmented_sort_ranges.restype = i
     lib.mgs_status_str.argtypes = [i]
     lib.mgs_status_str.restype = ctypes.c_char_p
     lib.mgs_last_hip_error.argtypes = []
@@ -180,3 +182,23 @@ def segmented_sort(kt, keys: torch.Tensor, offsets: torch.Tensor, vals: Optional
     st = lib.mgs_segmented_sort(kt, vt, _ptr(keys), _ptr(keys_out), _ptr(vals), _ptr(vals_out), _ptr(offsets),
                                 nseg, max_total, tp, _ptr(temp), temp.numel(), _stream(stream))
     _check(st, "mgs_segmented_sort")
+
+
+def segmented_sort_ranges(kt, keys: torch.Tensor, ranges: torch.Tensor, vals: Optional[torch.Tensor] = None,
+                          val_begin: Optional[torch.Tensor] = None, max_total: Optional[int] = None,
+                          tuning: Optional[Tuning] = None, temp=None, stream=None) -> None:
+    """In-place sort of segments keys[ranges[2s]:ranges[2s+1]] (device int64
+    tensor of 2*nseg bounds; segments disjoint, anywhere in keys); payloads
+    at vals[val_begin[s]:...] (default: the keys' positions)."""
+    kt = key_type(kt)
+    vt = VAL_NONE if vals is None else (VAL_U32 if vals.element_size() == 4 else VAL_U64)
+    nseg = ranges.numel() // 2
+    if max_total is None:
+        max_total = keys.numel()
+    need = lib.mgs_segmented_sort_temp_bytes(kt, vt, max_total, nseg)
+    if temp is None:
+        temp = temp_buffer(need, keys.device)
+    tp = ctypes.byref(tuning) if tuning is not None else None
+    st = lib.mgs_segmented_sort_ranges(kt, vt, _ptr(keys), _ptr(vals), _ptr(ranges), _ptr(val_begin), nseg, max_total, tp,
+                                       _ptr(temp), temp.numel(), _stream(stream))
+    _check(st, "mgs_segmented_sort_ranges")

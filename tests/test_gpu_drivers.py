@@ -1,0 +1,133 @@
+"""Drop-in drivers on the GPU: the single-device Tanasic sort (pivot, swap,
+merge of virtual partitions), the heterogeneous Gowanlock sort (chunks sorted
+on the GPU while the CPU merges) and the sort_benchmark CLI, each checked
+against the oracle's sort of the same reference-generated input.
+
+Reference drivers: src/tanasic_sort.cuh:168-259, src/gowanlock_sort.cuh:120-323,
+src/sort_benchmark.cu:64-228.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+HOST_LIB = os.path.join(ROOT, "multi-gpu-sorting_amd", "lib", "libmgsort_host.so")
+BENCH = os.path.join(ROOT, "multi-gpu-sorting_amd", "bin", "sort_benchmark")
+TYPES = {"int": (O.K_I32, np.int32), "long": (O.K_I64, np.int64), "float": (O.K_F32, np.float32),
+         "double": (O.K_F64, np.float64), "u32": (O.K_U32, np.uint32), "u64": (O.K_U64, np.uint64)}
+
+
+@pytest.fixture(scope="module")
+def H():
+    import mgsort  # noqa: F401  (loads libmgsort.so first, globally)
+    L = ctypes.CDLL(HOST_LIB)
+    c, vp, sz = ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t
+    L.mgh_generate.argtypes = [c, c, ctypes.c_uint32, vp, sz]
+    L.mgh_tanasic_sort.argtypes = [c, vp, sz, ctypes.c_int, sz, ctypes.POINTER(ctypes.c_uint64)]
+    L.mgh_gowanlock_sort.argtypes = [c, vp, sz, ctypes.c_int, sz, sz, sz, c, ctypes.c_int]
+    L.mgh_last_error.restype = c
+    L.mgh_phase_seconds.argtypes = [c]
+    L.mgh_phase_seconds.restype = ctypes.c_double
+    return L
+
+
+def generate(H, dtype, dist, seed, n):
+    kt, t = TYPES[dtype]
+    a = np.empty(n, dtype=t)
+    assert H.mgh_generate(dtype.encode(), dist.encode(), seed, a.ctypes.data, n) == 0
+    return a
+
+
+def expect_sorted_like_oracle(dtype, inp, got):
+    kt, _ = TYPES[dtype]
+    bits = O.BITS_DTYPE[kt]
+    exp, _ = O.sort(kt, inp.view(bits))
+    # keys only: equal encodings (-0.0 / +0.0) are interchangeable, so compare
+    # the encoded sequence and the multiset of raw bit patterns
+    assert np.array_equal(O.encode(kt, got.view(bits)), O.encode(kt, exp))
+    assert np.array_equal(np.sort(got.view(bits)), np.sort(inp.view(bits)))
+
+
+@pytest.mark.parametrize("dtype", ["int", "u32", "float", "long", "double"])
+@pytest.mark.parametrize("partitions", [2, 4, 8])
+@pytest.mark.parametrize("n", [1000, 1 << 16, 999_983])
+def test_tanasic_single_device(H, dtype, partitions, n):
+    a = generate(H, dtype, "uniform", 1234 + n, n)
+    k = a.copy()
+    piv = ctypes.c_uint64(0)
+    rc = H.mgh_tanasic_sort(dtype.encode(), k.ctypes.data, n, 0, partitions, ctypes.byref(piv))
+    assert rc == 0, H.mgh_last_error()
+    expect_sorted_like_oracle(dtype, a, k)
+
+
+@pytest.mark.parametrize("dist", ["sorted", "reverse-sorted", "zero", "nearly-sorted", "bucket-sorted"])
+def test_tanasic_distributions(H, dist):
+    n = 300_000
+    a = generate(H, "int", dist, 7, n)
+    k = a.copy()
+    assert H.mgh_tanasic_sort(b"int", k.ctypes.data, n, 0, 4, None) == 0, H.mgh_last_error()
+    expect_sorted_like_oracle("int", a, k)
+
+
+def test_tanasic_pivot_matches_oracle(H):
+    """The first pivot of a 2-partition sort equals the oracle's select_pivot
+    on the two independently sorted halves."""
+    n = 200_000
+    a = generate(H, "u32", "uniform", 99, n)
+    k = a.copy()
+    piv = ctypes.c_uint64(0)
+    assert H.mgh_tanasic_sort(b"u32", k.ctypes.data, n, 0, 2, ctypes.byref(piv)) == 0, H.mgh_last_error()
+    half = n // 2
+    left, _ = O.sort(O.K_U32, a[:half])
+    right, _ = O.sort(O.K_U32, a[half:])
+    assert piv.value == O.select_pivot(O.K_U32, [left], [right], half)
+    expect_sorted_like_oracle("u32", a, k)
+
+
+@pytest.mark.parametrize("backend", ["cpu", "gpu"])
+@pytest.mark.parametrize("chunk,group,buffers", [(0, 1, 2), (100_000, 1, 2), (100_000, 3, 3), (65_536, 2, 2),
+                                                 (1 << 20, 4, 3)])
+@pytest.mark.parametrize("dtype", ["int", "double"])
+def test_gowanlock(H, backend, chunk, group, buffers, dtype):
+    n = 777_777
+    a = generate(H, dtype, "uniform", 5 + chunk, n)
+    k = a.copy()
+    rc = H.mgh_gowanlock_sort(dtype.encode(), k.ctypes.data, n, 0, chunk, group, buffers, backend.encode(), 8)
+    assert rc == 0, H.mgh_last_error()
+    expect_sorted_like_oracle(dtype, a, k)
+
+
+def test_gowanlock_small_and_empty(H):
+    for n in (0, 1, 2, 17):
+        a = generate(H, "u32", "uniform", 3, n) if n else np.empty(0, np.uint32)
+        k = a.copy()
+        assert H.mgh_gowanlock_sort(b"u32", k.ctypes.data if n else None, n, 0, 8, 2, 2, b"cpu", 4) == 0
+        expect_sorted_like_oracle("u32", a, k)
+
+
+def run_bench(*args):
+    r = subprocess.run([BENCH, *map(str, args)], capture_output=True, text=True, timeout=300)
+    return r.returncode, r.stdout
+
+
+@pytest.mark.parametrize("args", [
+    (1 << 20, "--algorithm", "hip", "--data_type", "float"),
+    (1 << 20, "--algorithm", "thrust", "--data_type", "long", "--values", "u64"),
+    (1 << 20, "--algorithm", "tanasic", "--partitions", "4"),
+    (1 << 20, "--algorithm", "gowanlock", "--chunk_size", "100000", "--merge_group_size", "2", "--num_buffers", "3"),
+    (1 << 20, "--algorithm", "gowanlock", "--merge_backend", "gpu"),
+    (1 << 18, "--algorithm", "paradis", "--num_threads", "4", "--distribution_type", "zipf"),
+])
+def test_sort_benchmark_cli(args):
+    rc, out = run_bench(*args)
+    assert rc == 0, out
+    row = out.strip().splitlines()[-1].split(",")
+    assert row[-1] == "1"  # verified
+    assert int(row[0]) == args[0]

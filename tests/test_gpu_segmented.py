@@ -87,3 +87,91 @@ def test_segmented_out_of_place_with_base_offset(mg):
     got = from_dev(dout, np.uint32)
     assert np.array_equal(got[base:base + total], ek[base:base + total])
     assert np.array_equal(from_dev(din, np.uint32), buf)  # input untouched
+
+
+@pytest.mark.parametrize("kt", [O.K_U32, O.K_U64], ids=lambda k: KEY_NAME[k])
+@pytest.mark.parametrize("vdt", [None, np.uint32], ids=["keys", "pairs"])
+def test_segmented_ranges_scattered(mg, kt, vdt):
+    """Ranges form: disjoint segments at arbitrary, unordered places of one
+    array, sorted in place; the gaps between them stay untouched."""
+    import torch
+    rng = np.random.default_rng(21)
+    lengths = np.concatenate([[0, 1, 7, 2048, 2049, 70_000, 140_000, 300_000],
+                              np.round(2.0 ** (6 + 10 * rng.random(60))).astype(np.int64)])
+    gaps = rng.integers(0, 3000, size=len(lengths))
+    starts, pos = [], 0
+    for ln, g in zip(lengths, gaps):
+        pos += int(g)
+        starts.append(pos)
+        pos += int(ln)
+    n = pos + 100
+    buf = random_keys(kt, n, rng)
+    vals = None if vdt is None else np.arange(n, dtype=vdt)
+    order = rng.permutation(len(lengths))  # segments listed in shuffled order
+    ranges = np.array([[starts[i], starts[i] + lengths[i]] for i in order], dtype=np.uint64).reshape(-1)
+    ek, ev = buf.copy(), None if vals is None else vals.copy()
+    for i in range(len(lengths)):
+        a, b = starts[i], starts[i] + int(lengths[i])
+        k, v = O.sort(kt, buf[a:b], None if vals is None else vals[a:b])
+        ek[a:b] = k
+        if vals is not None:
+            ev[a:b] = v
+    dk, dv = to_dev(buf), to_dev(vals)
+    dr = torch.from_numpy(ranges.view(np.int64)).cuda()
+    mg.segmented_sort_ranges(KEY_NAME[kt], dk, dr, dv, max_total=int(lengths.sum()))
+    torch.cuda.synchronize()
+    assert np.array_equal(from_dev(dk, buf.dtype), ek)
+    if vals is not None:
+        assert np.array_equal(from_dev(dv, vdt), ev)
+
+
+def test_segmented_malformed_offsets_are_skipped(mg):
+    """A decreasing offset leaves the segments it would make overlap
+    unsorted, and no segment reaches past max_total; the rest sort."""
+    import torch
+    rng = np.random.default_rng(22)
+    n = 60_000
+    buf = random_keys(O.K_U32, n, rng)
+    # segments: [0,10000) ok, [10000,5000) bad, [5000,30000) starts before an
+    # earlier offset -> skipped, [30000,50000) ok, [50000,60000) beyond max_total
+    off = np.array([0, 10_000, 5_000, 30_000, 50_000, 60_000], dtype=np.uint64)
+    din = to_dev(buf)
+    mg.segmented_sort("u32", din, torch.from_numpy(off.view(np.int64)).cuda(), max_total=50_000)
+    torch.cuda.synchronize()
+    got = from_dev(din, np.uint32)
+    exp = buf.copy()
+    for a, b in [(0, 10_000), (30_000, 50_000)]:
+        exp[a:b] = O.sort(O.K_U32, buf[a:b])[0]
+    assert np.array_equal(got, exp)
+
+
+def test_segmented_ranges_separate_payload_places(mg):
+    """Payloads of a segment may sit elsewhere than its keys (val_begin)."""
+    import torch
+    rng = np.random.default_rng(23)
+    lengths = np.array([5, 3000, 150_000, 0, 70_000, 1])
+    n = int(lengths.sum())
+    kstart = np.concatenate([[0], np.cumsum(lengths)[:-1]])
+    vperm = rng.permutation(len(lengths))  # payload blocks stored in another order
+    vstart = np.zeros(len(lengths), dtype=np.int64)
+    pos = 0
+    for i in vperm:
+        vstart[i] = pos + 17
+        pos += int(lengths[i]) + 17
+    keys = random_keys(O.K_U64, n, rng, "few")
+    vals = np.zeros(pos + 10, dtype=np.uint32)
+    for i in range(len(lengths)):
+        vals[vstart[i]:vstart[i] + lengths[i]] = np.arange(lengths[i], dtype=np.uint32) + 1000 * i
+    ek, ev = keys.copy(), vals.copy()
+    for i in range(len(lengths)):
+        a, b = kstart[i], kstart[i] + lengths[i]
+        k, v = O.sort(O.K_U64, keys[a:b], vals[vstart[i]:vstart[i] + lengths[i]])
+        ek[a:b] = k
+        ev[vstart[i]:vstart[i] + lengths[i]] = v
+    ranges = np.stack([kstart, kstart + lengths], axis=1).reshape(-1).astype(np.int64)
+    dk, dv = to_dev(keys), to_dev(vals)
+    mg.segmented_sort_ranges("u64", dk, torch.from_numpy(ranges).cuda(), dv,
+                             val_begin=torch.from_numpy(vstart.astype(np.int64)).cuda(), max_total=n)
+    torch.cuda.synchronize()
+    assert np.array_equal(from_dev(dk, np.uint64), ek)
+    assert np.array_equal(from_dev(dv, np.uint32), ev)
