@@ -135,6 +135,35 @@ def batched_section():
     return out
 
 
+def serving_section(requests=192):
+    """Poisson stream of mixed requests (log-uniform 2^10..2^26 keys; 50% u32,
+    30% u64, 20% u32+u32 pairs; uniform / nearly sorted) through the serving
+    layer: continuous batching vs the plain per-request path."""
+    import subprocess
+    exe = os.path.join(ROOT, "multi-gpu-sorting_amd", "bin", "serve_benchmark")
+    p = subprocess.Popen([exe, "--requests", str(requests), "--budget_gib", "64", "--loads", "inf,2,1,0.5"],
+                         stdout=subprocess.PIPE, text=True)
+    rows = []
+    for line in p.stdout:  # progress as it comes
+        log("serve:", line.rstrip())
+        if line.startswith("{"):
+            rows.append(json.loads(line))
+    rc = p.wait(timeout=900)
+    if rc != 0:
+        raise RuntimeError(f"serve_benchmark failed ({rc})")
+    runs = [x for x in rows if "mode" in x]
+    out = {"workload": f"{requests} requests, sizes round(2^(10+16U)), mt19937_64 seed 7; 64 GiB HBM pool",
+           "runs": runs}
+    for data in ("device", "host"):
+        sat = {x["mode"]: x for x in runs if x["data"] == data and x["load"] == "inf"}
+        if "batched" in sat and "plain" in sat:
+            out[f"{data}_saturated_speedup_vs_plain"] = round(sat["batched"]["gkeys_s"] / sat["plain"]["gkeys_s"], 3)
+            out[f"{data}_saturated_speedup_vs_plain_overlap"] = round(
+                sat["batched"]["gkeys_s"] / sat["plain_overlap"]["gkeys_s"], 3)
+    out["same_as_alone"] = all(x["same_as_alone"] for x in runs)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +172,9 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 30)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched segmented-sort section")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only copy n u32 keys device to device (calibrates the profiler's byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the sort path of this build runs on one GPU "
@@ -152,6 +184,16 @@ def main():
     import numpy as np
     import torch
     import mgsort
+
+    if args.copy_only:
+        a = torch.empty(args.n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(args.warmup + args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_only": True, "n": args.n, "bytes_per_copy": 8 * args.n}), flush=True)
+        return
 
     H = host_lib()
     n = args.n
@@ -239,6 +281,8 @@ def main():
     if not args.no_batched:
         result["batched"] = batched_section()
         torch.cuda.empty_cache()
+    if not args.no_serving:
+        result["serving"] = serving_section()
     if not args.no_cpu:
         result.update(cpu_baseline(H, np))
     print(json.dumps(result), flush=True)

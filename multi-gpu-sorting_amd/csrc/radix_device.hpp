@@ -23,22 +23,28 @@ constexpr uint32_t kLookbackSpinLimit = 1u << 24;
 #define MGS_RANK(d, peers, cnt) rank_wave_atomic((d), (cnt))
 #endif
 
-// Look-back over predecessor tiles of one chain for one digit: sums
-// aggregates until an inclusive prefix is found.  `desc` points at the
-// chain's first tile.
+constexpr int kWindow = MGS_LB_WINDOW;
+
+// Issues the loads of the descriptors of predecessor tiles t, t-1, ...,
+// t-kWindow+1 (those >= 0) for digit d of a chain whose first tile's
+// descriptors start at `desc`.
 template <typename D>
-__device__ __forceinline__ uint32_t lookback(const D* desc, uint32_t tile, uint32_t d, int parity) {
-  // Reads kWindow predecessors per round trip (independent loads in flight),
-  // then consumes them in order.
-  constexpr int kWindow = MGS_LB_WINDOW;
-  uint32_t prefix = 0;
-  int64_t t = (int64_t)tile - 1;
-  uint32_t spins = 0;
-  for (;;) {
-    D v[kWindow];
+__device__ __forceinline__ void lookback_load(const D* desc, int64_t t, uint32_t d, D (&v)[kWindow]) {
 #pragma unroll
-    for (int j = 0; j < kWindow; ++j)
-      v[j] = (t - j >= 0) ? ld_agent(desc + (size_t)(t - j) * kRadix + d) : D(0);
+  for (int j = 0; j < kWindow; ++j) v[j] = (t - j >= 0) ? ld_agent(desc + (size_t)(t - j) * kRadix + d) : D(0);
+}
+
+// Look-back for one digit starting from a window already loaded at
+// predecessor t: sums aggregates, walking back, until an inclusive prefix is
+// found (or the chain's first tile is passed).  Issuing the first window
+// before the tile publishes its own aggregate keeps that store's completion
+// off the critical path.
+template <typename D>
+__device__ __forceinline__ uint32_t lookback_from(const D* desc, int64_t t, uint32_t d, int parity,
+                                                  D (&v)[kWindow], uint32_t* stats = nullptr) {
+  uint32_t prefix = 0;
+  uint32_t spins = 0, rounds = 1;
+  for (;;) {
     int consumed = 0;
     bool done = false;
 #pragma unroll
@@ -50,17 +56,29 @@ __device__ __forceinline__ uint32_t lookback(const D* desc, uint32_t tile, uint3
       consumed = j + 1;
       if (st == kStInclusive) done = true;
     }
-    if (done) break;
+    if (done || t - consumed < 0) break;
     if (consumed == 0) {
       // Bounded wait: a predecessor tile always publishes within microseconds;
       // the bound only keeps a corrupted descriptor array from hanging the GPU
-      // (the result is then wrong, every store stays bounds-checked).
-      if (++spins > kLookbackSpinLimit || t < 0) break;
+      // (the result is then wrong, every store stays in bounds).
+      if (++spins > kLookbackSpinLimit) break;
       __builtin_amdgcn_s_sleep(1);
     }
     t -= consumed;
+    lookback_load(desc, t, d, v);
+    ++rounds;
   }
+  if (stats) *stats = rounds | (spins << 16);
   return prefix;
+}
+
+// Look-back of tile `tile` (> 0) of a chain for digit d.
+template <typename D>
+__device__ __forceinline__ uint32_t lookback(const D* desc, uint32_t tile, uint32_t d, int parity,
+                                             uint32_t* stats = nullptr) {
+  D v[kWindow];
+  lookback_load(desc, (int64_t)tile - 1, d, v);
+  return lookback_from(desc, (int64_t)tile - 1, d, parity, v, stats);
 }
 
 // ---------------------------------------------------------------------------

@@ -307,13 +307,21 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
   __shared__ uint32_t off[W][kRadix];
   __shared__ uint32_t nh[kChains][kRadix];
   __shared__ uint32_t gbase[kRadix];
+  __shared__ uint32_t s_prefix[kRadix];
   __shared__ uint32_t sums[4];
   __shared__ uint32_t s_tile;
+  // With eight or more wavefronts the look-back loads run on waves 4..7 and
+  // the descriptor stores on waves 0..3, so no wavefront waits for its own
+  // store to complete before it can use a look-back load.
+  constexpr bool kSplitLookback = W >= 8;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int w = tid / kWave;
-  uint32_t my_chain = blockIdx.x % cm.chains, chains_left = cm.chains;  // thread 0's acquisition state
+  // Tiles are taken by one thread of a wavefront that issues no descriptor
+  // stores (its atomic then waits for nothing else); its acquisition state:
+  const int taker = kSplitLookback ? kRadix : 0;
+  uint32_t my_chain = blockIdx.x % cm.chains, chains_left = cm.chains;
 
   // Digit base of this byte (keys with a smaller digit), held by the thread
   // of each digit; the base of a chain adds the keys of earlier chains with
@@ -330,7 +338,7 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
 #ifdef MGS_RANK_PEERS
     for (int i = tid; i < W * kRadix; i += BLOCK) (&peers[0][0])[i] = 0;
 #endif
-    if (tid == 0) s_tile = take_tile(cnt, cm, my_chain, chains_left);
+    if (tid == taker) s_tile = take_tile(cnt, cm, my_chain, chains_left);
     __syncthreads();
   }
   uint32_t tile = s_tile;
@@ -373,15 +381,32 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
         pk[i] = (wb + i * kWave < n) ? ((d << 16) | MGS_RANK(d, pt, ct)) : 0u;
       }
     }
+    // The keys are in registers now; tell the compiler so, so that the
+    // scatter below does not wait on memory operations issued meanwhile.
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) asm volatile("" : "+v"(kc[i]));
     __syncthreads();
     PHASE(2);
 
     uint32_t total = 0;
     D* my_desc = desc + (size_t)tile * kRadix + tid;
+    const D* chain_desc = desc + (size_t)first * kRadix;
+    D win[kWindow];
     if (tid < kRadix) {
 #pragma unroll
       for (int i = 0; i < W; ++i) total += off[i][tid];
+      if (!kSplitLookback && lt > 0) lookback_load(chain_desc, (int64_t)lt - 1, (uint32_t)tid, win);
       st_agent(my_desc, Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)total, parity));
+    } else if (kSplitLookback && tid < 2 * kRadix) {
+      const uint32_t d = (uint32_t)tid - kRadix;
+#ifdef MGS_PHASE_PROBE
+      uint32_t st = 0;
+      s_prefix[d] = lt > 0 ? lookback(chain_desc, lt, d, parity, &st) : 0u;
+      if (d == 0 && g_phase_probe && (tile & 63) == 0)
+        g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 8 + 7] = st;
+#else
+      s_prefix[d] = lt > 0 ? lookback(chain_desc, lt, d, parity) : 0u;
+#endif
     }
     const uint32_t excl = block_excl_scan_256(total, sums, tid);
     if (tid < kRadix) {
@@ -394,7 +419,7 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
       }
       uint32_t prefix = 0;
       if (lt > 0) {
-        prefix = lookback(desc + (size_t)first * kRadix, lt, (uint32_t)tid, parity);
+        prefix = kSplitLookback ? s_prefix[tid] : lookback_from(chain_desc, (int64_t)lt - 1, (uint32_t)tid, parity, win);
         st_agent(my_desc, Desc<D>::make(kStInclusive, (D)(prefix + total), parity));
       }
       if (chain != cur_chain) {
@@ -411,7 +436,13 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
     // tile, so a tile's predecessor is always one that is already being
     // ranked (taking it earlier makes look-backs wait on tiles nobody works
     // on yet).
-    if (tid == 0) s_tile = take_tile(cnt, cm, my_chain, chains_left);
+    if (tid == taker) {
+      s_tile = take_tile(cnt, cm, my_chain, chains_left);
+#ifdef MGS_PHASE_PROBE
+      if (g_phase_probe && (tile & 63) == 0)
+        g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 8] = __builtin_amdgcn_s_memrealtime();
+#endif
+    }
 
 #pragma unroll
     for (int i = 0; i < IPT; ++i) {
@@ -439,12 +470,12 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
       const uint32_t i = (uint32_t)(j * BLOCK + tid);
       if (full || i < tile_cnt) {
         const Bits kk = keys[i];
-        const uint32_t o = gbase[digit_of<KT>(kk, shift)] + i;
-        if (o < n) {
-          dst[o] = kk;
-          if constexpr (HV) vdst[o] = vstage[i];
-          if (next >= 0) atomicAdd(&nh[chain_of(o, cm)][digit_of<KT>(kk, 8 * next)], 1u);
-        }
+        // o < n for every key of a well-formed pass; the clamp only keeps a
+        // corrupted prefix in bounds, and leaves the stores unconditional
+        const uint32_t o = min(gbase[digit_of<KT>(kk, shift)] + i, n - 1);
+        dst[o] = kk;
+        if constexpr (HV) vdst[o] = vstage[i];
+        if (next >= 0) atomicAdd(&nh[chain_of(o, cm)][digit_of<KT>(kk, 8 * next)], 1u);
       }
     }
     PHASE(5);

@@ -22,9 +22,9 @@ constexpr int kMaxPasses = 8;
 // Buffer ids of the ping-pong plan.
 enum : int32_t { kBufIn = 0, kBufAlt = 1, kBufOut = 2 };
 
-// Look-back chains per pass (one per XCD by default).
+// Look-back chains per pass (two per XCD by default).
 #ifndef MGS_CHAINS
-#define MGS_CHAINS 8
+#define MGS_CHAINS 16
 #endif
 constexpr int kChains = MGS_CHAINS;
 

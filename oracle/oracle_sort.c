@@ -20,9 +20,13 @@
  *   oracle_segmented_sort per-segment oracle_sort
  *
  * Parity pins (tests/test_oracle.py): oracle_sort against numpy's stable
- * argsort of the encoded keys and against brute-force insertion sort;
- * oracle_merge against a stable sort of the concatenation; the pivot against
- * a brute-force scan over all k and the invariant max(L') <= min(R').
+ * sort of the values themselves (integers, and floats with signed zeros,
+ * infinities and denormals, where -0.0 == +0.0 keep input order), numpy's
+ * stable argsort for payloads, a brute-force insertion sort, and the
+ * prescribed NaN / infinity placement; oracle_merge against a stable sort of
+ * the concatenation; the pivot against a brute-force scan over all k and the
+ * invariant max(L') <= min(R'); tanasic_1dev and segmented_sort against the
+ * sorted whole / numpy per segment.
  */
 #include <stdint.h>
 #include <stdlib.h>
