@@ -100,7 +100,7 @@ __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>
     const Bits e = Codec<KT>::enc(k);
     vor |= e;
     vand &= e;
-    atomicAdd(&h[(uint32_t)e & 0xFFu], 1u);
+    lds_count(h, (uint32_t)e & 0xFFu);
   };
   if constexpr (VEC) {
     // chain_keys is a multiple of the tile, so [lo, hi) starts 16-byte aligned
@@ -198,7 +198,7 @@ __device__ void histogram_only(const typename Codec<KT>::Bits* keys, uint32_t n,
   const size_t lo = (size_t)c * cm.chain_keys;
   const size_t hi = std::min<size_t>((size_t)n, lo + cm.chain_keys);
   for (size_t i = lo + (size_t)j * BLOCK + threadIdx.x; i < hi; i += (size_t)nb * BLOCK)
-    atomicAdd(&h[digit_of<KT>(keys[i], 8 * b)], 1u);
+    lds_count(h, digit_of<KT>(keys[i], 8 * b));
   __syncthreads();
   for (int i = threadIdx.x; i < kRadix; i += BLOCK)
     if (h[i]) atomicAdd(&hist_b[c * kRadix + i], h[i]);
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
   // With eight or more wavefronts the look-back loads run on waves 4..7 and
   // the descriptor stores on waves 0..3, so no wavefront waits for its own
   // store to complete before it can use a look-back load.
-  constexpr bool kSplitLookback = W >= 8;
+  constexpr bool kSplitLookback = MGS_SPLIT_LB && W >= 8;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
@@ -329,8 +329,11 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
   uint32_t dbase = 0, cur_chain = 0xFFFFFFFFu, cur_base = 0;
   {
     uint32_t tot = 0;
-    if (tid < kRadix)
-      for (uint32_t c = 0; c < cm.chains; ++c) tot += hp[c * kRadix + tid];
+    if (tid < kRadix) {
+#pragma unroll
+      for (uint32_t c = 0; c < (uint32_t)kChains; ++c)
+        if (c < cm.chains) tot += hp[c * kRadix + tid];  // independent loads, all in flight
+    }
     dbase = block_excl_scan_256(tot, sums, tid);
     if (tid < kRadix)
       for (uint32_t c = 0; c < (uint32_t)kChains; ++c) nh[c][tid] = 0;
@@ -369,11 +372,18 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
 #endif
     (void)pt;
     if (full) {
+#ifdef MGS_RANK_PEERS
 #pragma unroll
       for (int i = 0; i < IPT; ++i) {
         const uint32_t d = digit_of<KT>(kc[i], shift);
         pk[i] = (d << 16) | MGS_RANK(d, pt, ct);
       }
+#else
+      uint32_t dg[IPT];
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) dg[i] = digit_of<KT>(kc[i], shift);
+      rank_items_full<IPT>(dg, ct, pk);
+#endif
     } else {
 #pragma unroll
       for (int i = 0; i < IPT; ++i) {
@@ -425,7 +435,9 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
       if (chain != cur_chain) {
         cur_chain = chain;
         cur_base = dbase;
-        for (uint32_t c = 0; c < chain; ++c) cur_base += hp[c * kRadix + tid];
+#pragma unroll
+        for (uint32_t c = 0; c < (uint32_t)kChains; ++c)
+          if (c < chain) cur_base += hp[c * kRadix + tid];
       }
       gbase[tid] = cur_base + prefix - excl;
       if (tid == 0) PHASE(4);
@@ -465,17 +477,48 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
     }
     for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
 
+    // Write-out: consecutive threads store consecutive positions of each
+    // digit run.  o < n for every key of a well-formed pass; the clamp only
+    // keeps a corrupted prefix in bounds and leaves the stores unconditional.
+    // Full tiles go in batches of G items, LDS reads of a batch issued
+    // together, then the stores, then the counting for the next byte.
+    if (MGS_BATCH_WO && full) {
+      constexpr int G = IPT < 8 ? IPT : 8;
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint32_t i = (uint32_t)(j * BLOCK + tid);
-      if (full || i < tile_cnt) {
-        const Bits kk = keys[i];
-        // o < n for every key of a well-formed pass; the clamp only keeps a
-        // corrupted prefix in bounds, and leaves the stores unconditional
-        const uint32_t o = min(gbase[digit_of<KT>(kk, shift)] + i, n - 1);
-        dst[o] = kk;
-        if constexpr (HV) vdst[o] = vstage[i];
-        if (next >= 0) atomicAdd(&nh[chain_of(o, cm)][digit_of<KT>(kk, 8 * next)], 1u);
+      for (int j0 = 0; j0 < IPT; j0 += G) {
+        Bits kk[G];
+        VS vv[G];
+        uint32_t oo[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          kk[g] = keys[(j0 + g) * BLOCK + tid];
+          if constexpr (HV) vv[g] = vstage[(j0 + g) * BLOCK + tid];
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+          oo[g] = min(gbase[digit_of<KT>(kk[g], shift)] + (uint32_t)((j0 + g) * BLOCK + tid), n - 1);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          dst[oo[g]] = kk[g];
+          if constexpr (HV) vdst[oo[g]] = vv[g];
+        }
+        if (next >= 0) {
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+            lds_count(&nh[0][0], chain_of(oo[g], cm) * kRadix + digit_of<KT>(kk[g], 8 * next));
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t i = (uint32_t)(j * BLOCK + tid);
+        if (full || i < tile_cnt) {
+          const Bits kk = keys[i];
+          const uint32_t o = min(gbase[digit_of<KT>(kk, shift)] + i, n - 1);
+          dst[o] = kk;
+          if constexpr (HV) vdst[o] = vstage[i];
+          if (next >= 0) lds_count(&nh[0][0], chain_of(o, cm) * kRadix + digit_of<KT>(kk, 8 * next));
+        }
       }
     }
     PHASE(5);
@@ -487,6 +530,290 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
       kc[i] = kn[i];
       if constexpr (HV) vc[i] = vn[i];
     }
+  }
+  if (next >= 0 && tid < kRadix) {
+    uint32_t* hn = hist + (size_t)next * kChains * kRadix;
+    for (uint32_t c = 0; c < cm.chains; ++c)
+      if (nh[c][tid]) atomicAdd(&hn[c * kRadix + tid], nh[c][tid]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One onesweep pass in chunks of CT tiles, persistent workgroups.
+//
+// A workgroup takes a chunk (CT consecutive tiles of one chain) and first
+// counts the digits of the whole chunk (phase H, last tile first so that the
+// first tile's keys stay in registers), publishes the chunk's counts and
+// issues the look-back loads.  It then ranks, scatters and writes the tiles
+// one after another with running per-digit offsets (phase S); the look-back
+// result is needed only by the first write-out, so its latency is spent
+// ranking and scattering the first tile, and there is one look-back per
+// chunk instead of one per tile.  The later tiles of the chunk are read a
+// second time a few microseconds after phase H (from the caches in the usual
+// case).
+// ---------------------------------------------------------------------------
+#ifndef MGS_CHUNK_TILES
+#define MGS_CHUNK_TILES 1
+#endif
+
+template <int KT, int IPT>
+__device__ __forceinline__ void load_keys(const typename Codec<KT>::Bits* __restrict__ src, uint32_t n, uint32_t wb,
+                                          bool full, typename Codec<KT>::Bits (&k)[IPT]) {
+  using Bits = typename Codec<KT>::Bits;
+  if (full) {
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) k[i] = src[wb + i * kWave];
+  } else {
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const uint32_t idx = wb + i * kWave;
+      k[i] = idx < n ? src[idx] : Bits(0);
+    }
+  }
+}
+
+template <int KT, typename V, int BLOCK, int IPT, int CT, typename D>
+__global__ __launch_bounds__(BLOCK, BLOCK / 128) void chunk_pass_kernel(
+    const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
+    const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, ChainMap cm,
+    const uint64_t* __restrict__ red, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters,
+    D* __restrict__ desc) {
+  using Bits = typename Codec<KT>::Bits;
+  constexpr int P = (int)sizeof(Bits);
+  constexpr int W = BLOCK / kWave;
+  constexpr int TILE = BLOCK * IPT;
+  constexpr uint32_t CHUNK = (uint32_t)TILE * CT;
+  constexpr bool HV = kHasVal<V>;
+  using VS = typename std::conditional<HV, V, uint32_t>::type;
+  static_assert(W >= 8, "the look-back runs on waves 4..7");
+
+  const uint32_t vary = varying_bytes(red, P);
+  if (!(vary & (1u << pass))) {
+    const uint32_t below = vary & ((2u << pass) - 1u);
+    if (below == 0 && pass + 1 < P && (vary & (1u << (pass + 1))))
+      histogram_only<KT, BLOCK>(k_in, n, pass + 1, hist + (size_t)(pass + 1) * kChains * kRadix, cm);
+    return;
+  }
+  const int k = __builtin_popcount(vary & ((1u << pass) - 1u));
+  const int na = __builtin_popcount(vary);
+  const uint32_t above = vary & ~((2u << pass) - 1u);
+  const int next = above ? __builtin_ctz(above) : -1;
+  const int sb = src_of(k, na, aliased), db = dst_of(k, na, aliased);
+  const Bits* __restrict__ src = pick_src(sb, k_in, (const Bits*)k_alt, (const Bits*)k_out);
+  Bits* __restrict__ dst = pick_dst(db, k_alt, k_out);
+  const V* __restrict__ vsrc = nullptr;
+  V* __restrict__ vdst = nullptr;
+  if constexpr (HV) {
+    vsrc = pick_src(sb, v_in, (const V*)v_alt, (const V*)v_out);
+    vdst = pick_dst(db, v_alt, v_out);
+  }
+  const int shift = 8 * pass, parity = k & 1;
+  uint32_t* __restrict__ cnt = counters + (size_t)pass * kChains;
+  const uint32_t* __restrict__ hp = hist + (size_t)pass * kChains * kRadix;
+
+  __shared__ Bits keys[TILE];
+  __shared__ VS vstage[HV ? TILE : 1];
+  __shared__ uint32_t off[W][kRadix];   // rank counters of the current tile
+  __shared__ uint32_t hcnt[W][kRadix];  // digit counts of the current chunk
+  __shared__ uint32_t nh[kChains][kRadix];
+  __shared__ uint32_t gbase[kRadix];
+  __shared__ uint32_t s_prefix[kRadix];
+  __shared__ uint32_t sums[4];
+  __shared__ uint32_t s_chunk;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int w = tid / kWave;
+  const int taker = kRadix;  // wave 4: issues no descriptor stores
+  uint32_t my_chain = blockIdx.x % cm.chains, chains_left = cm.chains;
+
+  uint32_t dbase = 0, cur_chain = 0xFFFFFFFFu, cur_base = 0;
+  {
+    uint32_t tot = 0;
+    if (tid < kRadix) {
+#pragma unroll
+      for (uint32_t c = 0; c < (uint32_t)kChains; ++c)
+        if (c < cm.chains) tot += hp[c * kRadix + tid];
+    }
+    dbase = block_excl_scan_256(tot, sums, tid);
+    if (tid < kRadix)
+      for (uint32_t c = 0; c < (uint32_t)kChains; ++c) nh[c][tid] = 0;
+    for (int i = tid; i < W * kRadix; i += BLOCK) {
+      (&off[0][0])[i] = 0;
+      (&hcnt[0][0])[i] = 0;
+    }
+    if (tid == taker) s_chunk = take_tile(cnt, cm, my_chain, chains_left);
+    __syncthreads();
+  }
+  uint32_t chunk = s_chunk;
+
+  while (chunk != kNoTile) {
+    const uint32_t chain = chunk / cm.tpc;
+    const uint32_t first = chain * cm.tpc;
+    const uint32_t lc = chunk - first;  // chunk index inside its chain
+    const uint32_t cb = chunk * CHUNK;
+    const uint32_t ccnt = min(CHUNK, n - cb);
+    const uint32_t ntiles = (ccnt + TILE - 1) / TILE;
+    const uint32_t lw = (uint32_t)(w * IPT * kWave + lane);
+
+    // ---- phase H: digit counts of the chunk, last tile first; the loads of
+    // the next tile down are issued before the current one is counted (a
+    // runtime loop, so the compiler keeps two tiles in registers, not CT) ----
+    Bits kc[IPT], kp[IPT];
+    VS vc[IPT];
+    {
+      int tt = (int)ntiles - 1;
+      uint32_t tb = cb + (uint32_t)tt * TILE;
+      bool full = n - tb >= (uint32_t)TILE;
+      if (tt == 0)
+        load_tile<KT, V, IPT>(src, vsrc, n, tb + lw, full, kc, vc);
+      else
+        load_keys<KT, IPT>(src, n, tb + lw, full, kc);
+#pragma unroll 1
+      for (; tt >= 0; --tt) {
+        if (tt >= 1) {  // tiles below the chunk's last are full
+          if (tt == 1)
+            load_tile<KT, V, IPT>(src, vsrc, n, tb - TILE + lw, true, kp, vc);
+          else
+            load_keys<KT, IPT>(src, n, tb - TILE + lw, true, kp);
+        }
+#pragma unroll
+        for (int i = 0; i < IPT; ++i)
+          if (full || tb + lw + i * kWave < n) atomicAdd(&hcnt[w][digit_of<KT>(kc[i], shift)], 1u);
+        if (tt >= 1) {
+#pragma unroll
+          for (int i = 0; i < IPT; ++i) kc[i] = kp[i];
+          tb -= TILE;
+          full = true;
+        }
+      }
+    }
+    __syncthreads();
+
+    // publish the chunk's counts; waves 4..7 issue the look-back loads
+    uint32_t ctot = 0;
+    D* my_desc = desc + (size_t)chunk * kRadix + tid;
+    const D* chain_desc = desc + (size_t)first * kRadix;
+    D win[kWindow];
+    if (tid < kRadix) {
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        ctot += hcnt[i][tid];
+        hcnt[i][tid] = 0;  // ready for the next chunk (read only here)
+      }
+      st_agent(my_desc, Desc<D>::make(lc == 0 ? kStInclusive : kStAggregate, (D)ctot, parity));
+      if (chain != cur_chain) {
+        cur_chain = chain;
+        cur_base = dbase;
+#pragma unroll
+        for (uint32_t c = 0; c < (uint32_t)kChains; ++c)
+          if (c < chain) cur_base += hp[c * kRadix + tid];
+      }
+    } else if (tid < 2 * kRadix) {
+      if (lc > 0) lookback_load(chain_desc, (int64_t)lc - 1, (uint32_t)(tid - kRadix), win);
+    }
+
+    // ---- phase S: tiles in order, running offsets per digit ----
+    uint32_t run = 0;  // per digit (tid < 256): next free output slot
+    for (uint32_t tt = 0; tt < ntiles; ++tt) {
+      const uint32_t tb = cb + tt * TILE;
+      const uint32_t tile_cnt = min((uint32_t)TILE, n - tb);
+      const bool full = tile_cnt == (uint32_t)TILE;
+      const uint32_t wb = tb + lw;
+      uint32_t pk[IPT];
+      uint32_t* ct = off[w];
+      if (full) {
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+          const uint32_t d = digit_of<KT>(kc[i], shift);
+          pk[i] = (d << 16) | __hip_atomic_fetch_add(&ct[d], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+          const uint32_t d = digit_of<KT>(kc[i], shift);
+          pk[i] = (wb + i * kWave < n) ? ((d << 16) | rank_wave_atomic(d, ct)) : 0u;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) asm volatile("" : "+v"(kc[i]));
+      const bool last = tt + 1 == ntiles;
+      if (last && tid == taker) s_chunk = take_tile(cnt, cm, my_chain, chains_left);
+      __syncthreads();
+
+      uint32_t total = 0;
+      if (tid < kRadix) {
+#pragma unroll
+        for (int i = 0; i < W; ++i) total += off[i][tid];
+      }
+      const uint32_t excl = block_excl_scan_256(total, sums, tid);
+      if (tid < kRadix) {
+        uint32_t r = excl;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+          const uint32_t c = off[i][tid];
+          off[i][tid] = r;
+          r += c;
+        }
+      }
+      if (tt == 0) {
+        // scatter first, then finish the look-back and publish the prefix
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+          const uint32_t pos = off[w][pk[i] >> 16] + (pk[i] & 0xFFFFu);
+          if ((full || wb + i * kWave < n) && pos < (uint32_t)TILE) {
+            keys[pos] = kc[i];
+            if constexpr (HV) vstage[pos] = vc[i];
+          }
+        }
+        if (tid >= kRadix && tid < 2 * kRadix)
+          s_prefix[tid - kRadix] = lc > 0 ? lookback_from(chain_desc, (int64_t)lc - 1, (uint32_t)(tid - kRadix), parity, win) : 0u;
+        __syncthreads();
+        if (tid < kRadix) {
+          const uint32_t prefix = s_prefix[tid];
+          if (lc > 0) st_agent(my_desc, Desc<D>::make(kStInclusive, (D)(prefix + ctot), parity));
+          run = cur_base + prefix;
+          gbase[tid] = run - excl;
+          run += total;
+        }
+      } else {
+        if (tid < kRadix) {
+          gbase[tid] = run - excl;
+          run += total;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+          const uint32_t pos = off[w][pk[i] >> 16] + (pk[i] & 0xFFFFu);
+          if ((full || wb + i * kWave < n) && pos < (uint32_t)TILE) {
+            keys[pos] = kc[i];
+            if constexpr (HV) vstage[pos] = vc[i];
+          }
+        }
+      }
+      __syncthreads();
+
+      // prefetch the chunk's next tile while this one is written out
+      if (!last) {
+        const uint32_t nb = tb + TILE;
+        load_tile<KT, V, IPT>(src, vsrc, n, nb + lw, n - nb >= (uint32_t)TILE, kc, vc);
+      }
+      for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t i = (uint32_t)(j * BLOCK + tid);
+        if (full || i < tile_cnt) {
+          const Bits kk = keys[i];
+          const uint32_t o = min(gbase[digit_of<KT>(kk, shift)] + i, n - 1);
+          dst[o] = kk;
+          if constexpr (HV) vdst[o] = vstage[i];
+          if (next >= 0) lds_count(&nh[0][0], chain_of(o, cm) * kRadix + digit_of<KT>(kk, 8 * next));
+        }
+      }
+      __syncthreads();
+    }
+    chunk = s_chunk;
   }
   if (next >= 0 && tid < kRadix) {
     uint32_t* hn = hist + (size_t)next * kChains * kRadix;
@@ -544,11 +871,20 @@ struct SortLayout {
   bool wide_desc;
 };
 
+// Keys per look-back unit of a pass: a chunk of MGS_CHUNK_TILES tiles when
+// the tile has the eight wavefronts the chunked kernel needs, else a tile.
+template <int KB, int VB>
+struct PassUnit {
+  using Cfg = TileCfg<KB, VB>;
+  static constexpr int kChunkTiles = (MGS_CHUNK_TILES > 1 && Cfg::kBlock >= 512) ? MGS_CHUNK_TILES : 1;
+  static constexpr size_t kKeys = (size_t)Cfg::kTile * kChunkTiles;
+};
+
 template <int KB, int VB>
 SortLayout sort_layout(size_t n) {
-  using Cfg = TileCfg<KB, VB>;
+  constexpr size_t unit = PassUnit<KB, VB>::kKeys;
   SortLayout L{};
-  L.num_tiles = (uint32_t)((n + Cfg::kTile - 1) / Cfg::kTile);
+  L.num_tiles = (uint32_t)((n + unit - 1) / unit);
   L.wide_desc = n > (size_t(1) << 30);
   size_t o = 0;
   auto take = [&](size_t bytes) {
@@ -637,7 +973,8 @@ int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t 
   const size_t desc_words = (size_t)L.num_tiles * kRadix * (L.wide_desc ? 2 : 1);
   const int aliased = (kin == kout) || (kHasVal<V> && vin == vout);
 
-  const ChainMap cm = make_chain_map(n, Cfg::kTile, kChains);
+  using PU = PassUnit<KB, VB>;
+  const ChainMap cm = make_chain_map(n, (uint32_t)PU::kKeys, kChains);
 
   mark(s);
   init_kernel<<<1, 1024, 0, s>>>(hist, (uint32_t)(P * kChains * kRadix), counters, (uint32_t)(P * kChains), red);
@@ -662,10 +999,17 @@ int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t 
   };
   for (int p = 0; p < P; ++p) {
     mark(s);
-    if (L.wide_desc)
-      launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint64_t>, (uint64_t*)desc, p);
-    else
-      launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint32_t>, (uint32_t*)desc, p);
+    if constexpr (PU::kChunkTiles > 1) {
+      if (L.wide_desc)
+        launch_pass(chunk_pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, PU::kChunkTiles, uint64_t>, (uint64_t*)desc, p);
+      else
+        launch_pass(chunk_pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, PU::kChunkTiles, uint32_t>, (uint32_t*)desc, p);
+    } else {
+      if (L.wide_desc)
+        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint64_t>, (uint64_t*)desc, p);
+      else
+        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint32_t>, (uint32_t*)desc, p);
+    }
   }
   mark(s);
   {

@@ -145,8 +145,91 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 // the conflicting lanes of one instruction in ascending lane order (measured
 // on gfx950 over 786k random conflict patterns; every multi-pass parity test
 // with payloads re-checks it, since any other order breaks LSD stability).
+// Tuning switches of the pass kernel (measured on MI355X; see the launch
+// sequence in radix_sort.hip):
+//   MGS_UNI        uniform-digit fast path in ranking and counting
+//   MGS_BATCH_WO   write-out of full tiles in batches of 8 items
+//   MGS_SPLIT_LB   look-back loads on waves 4..7, descriptor stores on 0..3
+#ifndef MGS_UNI
+#define MGS_UNI 0
+#endif
+#ifndef MGS_BATCH_WO
+#define MGS_BATCH_WO 0
+#endif
+#ifndef MGS_SPLIT_LB
+#define MGS_SPLIT_LB 1
+#endif
+
+// A wavefront whose active lanes all carry one digit (runs of equal bytes in
+// sorted or nearly sorted input) takes one atomic for all of them instead of
+// 64 conflicting ones.  Branch-free, so that the atomics of consecutive items
+// issue back to back: in the uniform case the first lane adds the lane count
+// and the others add 0 to a private slot of `sink`.
 __device__ __forceinline__ uint32_t rank_wave_atomic(uint32_t d, uint32_t* cnt) {
+#if !MGS_UNI
   return __hip_atomic_fetch_add(&cnt[d], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#else
+  __shared__ uint32_t sink[kWave];
+  const uint64_t act = ballot64(true);
+  const uint32_t d0 = readfirstlane(d);
+  const bool uni = ballot64(d == d0) == act;
+  const uint32_t below = mbcnt64(act);
+  const bool lead = !uni || below == 0;
+  const uint32_t inc = uni ? (below == 0 ? (uint32_t)__popcll(act) : 0u) : 1u;
+  const uint32_t r = __hip_atomic_fetch_add(lead ? &cnt[d] : &sink[lane_id()], inc, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+  const uint32_t r0 = readfirstlane(r);
+  return uni ? r0 + below : r;
+#endif
+}
+
+// rank_wave_atomic for IPT items of a full wavefront at once: all atomics
+// are issued first, the uniform-digit fix-ups follow, so the LDS round trips
+// overlap.  Returns digit << 16 | rank per item.
+template <int IPT>
+__device__ __forceinline__ void rank_items_full(const uint32_t (&d)[IPT], uint32_t* cnt, uint32_t (&pk)[IPT]) {
+#if !MGS_UNI
+#pragma unroll
+  for (int i = 0; i < IPT; ++i)
+    pk[i] = (d[i] << 16) | __hip_atomic_fetch_add(&cnt[d[i]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#else
+  __shared__ uint32_t sink[kWave];
+  const uint32_t lane = lane_id();
+  uint32_t r[IPT];
+  uint32_t uni_mask = 0;  // wave-uniform
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const uint32_t d0 = readfirstlane(d[i]);
+    const bool uni = ballot64(d[i] == d0) == ~0ull;
+    uni_mask |= uni ? (1u << i) : 0u;
+    const bool lead = !uni || lane == 0;
+    const uint32_t inc = uni ? (lane == 0 ? (uint32_t)kWave : 0u) : 1u;
+    r[i] = __hip_atomic_fetch_add(lead ? &cnt[d[i]] : &sink[lane], inc, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const uint32_t rk = ((uni_mask >> i) & 1u) ? readfirstlane(r[i]) + lane : r[i];
+    pk[i] = (d[i] << 16) | rk;
+  }
+#endif
+}
+
+// Counts one item per active lane into LDS counter c[i]; when every active
+// lane counts into the same counter, one atomic adds them all (same
+// branch-free scheme as rank_wave_atomic).
+__device__ __forceinline__ void lds_count(uint32_t* c, uint32_t i) {
+#if !MGS_UNI
+  atomicAdd(&c[i], 1u);
+#else
+  __shared__ uint32_t sink[kWave];
+  const uint64_t act = ballot64(true);
+  const uint32_t i0 = readfirstlane(i);
+  const bool uni = ballot64(i == i0) == act;
+  const bool lead = !uni || mbcnt64(act) == 0;
+  const uint32_t inc = uni ? (lead ? (uint32_t)__popcll(act) : 0u) : 1u;
+  __hip_atomic_fetch_add(lead ? &c[i] : &sink[lane_id()], inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#endif
 }
 
 // Exclusive scan of one value per digit held by threads 0..255 (4 waves).
