@@ -129,6 +129,15 @@ __device__ __forceinline__ void st_agent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Index of the XCD (one of the accelerator dies, each with its own L2) the
+// wavefront runs on.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcc_id() {
+  uint32_t v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return v & 0xFu;
+}
+
 // Workgroup-scope relaxed atomics on LDS tables that lanes of one wavefront
 // update and re-read within an instruction stream (see rank_wave()).
 template <typename T>

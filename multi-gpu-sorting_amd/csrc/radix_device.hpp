@@ -8,11 +8,19 @@
 
 namespace mgs {
 
-constexpr uint32_t kLookbackSpinLimit = 1u << 24;
+#ifndef MGS_LB_SPIN_LIMIT
+#define MGS_LB_SPIN_LIMIT (1u << 20)
+#endif
+constexpr uint32_t kLookbackSpinLimit = MGS_LB_SPIN_LIMIT;
 
-// Predecessor descriptors read per look-back round trip.
+// Predecessor descriptors read per look-back round trip: segmented sort
+// tiles, and the LSD pass kernel (whose window registers stay live across
+// the digit scan and the scatter).
 #ifndef MGS_LB_WINDOW
 #define MGS_LB_WINDOW 8
+#endif
+#ifndef MGS_PASS_LB_WINDOW
+#define MGS_PASS_LB_WINDOW 4
 #endif
 
 // Wavefront ranking used by every kernel: one LDS atomic per item by
@@ -24,14 +32,15 @@ constexpr uint32_t kLookbackSpinLimit = 1u << 24;
 #endif
 
 constexpr int kWindow = MGS_LB_WINDOW;
+constexpr int kPassWindow = MGS_PASS_LB_WINDOW;
 
 // Issues the loads of the descriptors of predecessor tiles t, t-1, ...,
-// t-kWindow+1 (those >= 0) for digit d of a chain whose first tile's
+// t-NW+1 (those >= 0) for digit d of a chain whose first tile's
 // descriptors start at `desc`.
-template <typename D>
-__device__ __forceinline__ void lookback_load(const D* desc, int64_t t, uint32_t d, D (&v)[kWindow]) {
+template <typename D, int NW>
+__device__ __forceinline__ void lookback_load(const D* desc, int64_t t, uint32_t d, D (&v)[NW]) {
 #pragma unroll
-  for (int j = 0; j < kWindow; ++j) v[j] = (t - j >= 0) ? ld_agent(desc + (size_t)(t - j) * kRadix + d) : D(0);
+  for (int j = 0; j < NW; ++j) v[j] = (t - j >= 0) ? ld_agent(desc + (size_t)(t - j) * kRadix + d) : D(0);
 }
 
 // Look-back for one digit starting from a window already loaded at
@@ -39,16 +48,16 @@ __device__ __forceinline__ void lookback_load(const D* desc, int64_t t, uint32_t
 // found (or the chain's first tile is passed).  Issuing the first window
 // before the tile publishes its own aggregate keeps that store's completion
 // off the critical path.
-template <typename D>
+template <typename D, int NW>
 __device__ __forceinline__ uint32_t lookback_from(const D* desc, int64_t t, uint32_t d, int parity,
-                                                  D (&v)[kWindow], uint32_t* stats = nullptr) {
+                                                  D (&v)[NW], uint32_t* stats = nullptr) {
   uint32_t prefix = 0;
   uint32_t spins = 0, rounds = 1;
   for (;;) {
     int consumed = 0;
     bool done = false;
 #pragma unroll
-    for (int j = 0; j < kWindow; ++j) {
+    for (int j = 0; j < NW; ++j) {
       if (done || consumed != j || t - j < 0) continue;
       const uint32_t st = Desc<D>::status(v[j], parity);
       if (st == kStEmpty) continue;

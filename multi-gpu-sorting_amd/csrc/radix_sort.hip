@@ -45,6 +45,7 @@ __global__ void init_kernel(uint32_t* __restrict__ hist, uint32_t words, uint32_
   if (g == 0) {
     red[0] = 0;     // OR identity
     red[1] = ~0ull;  // AND identity
+    red[2] = 0;      // XCDs seen by the upfront kernel
   }
 }
 
@@ -152,6 +153,7 @@ __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>
     atomicOr(reinterpret_cast<unsigned long long*>(&red[0]), (unsigned long long)a);
     atomicAnd(reinterpret_cast<unsigned long long*>(&red[1]),
               (unsigned long long)b | (sizeof(Bits) == 4 ? 0xFFFFFFFF00000000ull : 0ull));
+    atomicOr(reinterpret_cast<unsigned long long*>(&red[2]), 1ull << xcc_id());
   }
   for (int i = tid; i < kRadix; i += BLOCK) {
     const uint32_t v = h[i];
@@ -228,18 +230,39 @@ __device__ __forceinline__ void load_tile(const typename Codec<KT>::Bits* __rest
 }
 
 // Next tile for a workgroup: tiles of its current chain in order, then of the
-// following chains once that one is used up.  Thread-private state: the
-// current chain and the number of chains not yet found empty.
+// chain `step` further on (modulo the chain count) once that one is used up.
+// Thread-private state: the current chain and the number of chains of its
+// group not yet found empty.
 __device__ __forceinline__ uint32_t take_tile(uint32_t* __restrict__ cnt, const ChainMap& cm, uint32_t& c,
-                                              uint32_t& left) {
+                                              uint32_t& left, uint32_t step = 1) {
   while (left > 0) {
     const uint32_t t = atomicAdd(&cnt[c], 1u);
     const uint32_t first = c * cm.tpc;
     if (t < min(cm.tpc, cm.tiles - first)) return first + t;
-    c = (c + 1 == cm.chains) ? 0u : c + 1;
+    c += step;
+    if (c >= cm.chains) c -= cm.chains;
     --left;
   }
   return kNoTile;
+}
+
+// XCD-local look-back.  Each XCD has its own L2.  Measured on MI355X with a
+// ping-pong between workgroups: a relaxed agent-scope load sees a
+// workgroup-scope store of another workgroup on the same XCD (the store stays
+// in that L2, the load is served from it: 0.56 us round trip, against
+// 0.99 us when the store is agent-scope as well), but a workgroup on another
+// XCD does not see it before the kernel ends.  So when the upfront kernel has
+// run on all eight XCDs (ids 0..7) and the chain count is a multiple of
+// eight, chain c is served only by workgroups running on XCD c mod 8, and
+// its descriptors are stored with workgroup scope; every tile of a chain is
+// then written and read through one L2.  Otherwise descriptors are stored
+// with agent scope and any workgroup may serve any chain.
+template <typename D>
+__device__ __forceinline__ void publish(D* p, D v, bool xcd_local) {
+  if (xcd_local)
+    st_wg(p, v);
+  else
+    st_agent(p, v);
 }
 
 #ifdef MGS_PHASE_PROBE
@@ -247,7 +270,7 @@ __device__ uint64_t* g_phase_probe;
 #define PHASE(slot)                                                                                 \
   do {                                                                                              \
     if (g_phase_probe && tid == 0 && (tile & 63) == 0)                                              \
-      g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 8 + (slot)] = __builtin_amdgcn_s_memrealtime(); \
+      g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 16 + (slot)] = __builtin_amdgcn_s_memrealtime(); \
   } while (0)
 #else
 #define PHASE(slot) \
@@ -259,8 +282,32 @@ __device__ uint64_t* g_phase_probe;
 // One onesweep pass over byte `pass`, persistent workgroups, chained tiles.
 // hist is [P][kChains][256]; counters is [P][kChains].
 // ---------------------------------------------------------------------------
+// Wavefronts per SIMD the pass kernel is compiled for (caps its VGPRs):
+// two 512-thread workgroups per CU by default.
+// Digit base of a chain: `dbase` (keys with a smaller digit) plus the keys of
+// earlier chains with the same digit.  The row offset is made opaque so the
+// compiler does not hold the per-chain addresses in registers across the
+// tile loop (this runs only when a workgroup changes chain); the loads are
+// unconditional (every chain row exists) so that all are in flight at once.
+__device__ __forceinline__ uint32_t chain_digit_base(const uint32_t* __restrict__ hp, uint32_t dbase, uint32_t chain,
+                                                     int tid) {
+  uint32_t o = (uint32_t)tid;
+  asm volatile("" : "+v"(o));
+  uint32_t h[kChains];
+#pragma unroll
+  for (uint32_t c = 0; c < (uint32_t)kChains; ++c) h[c] = hp[c * kRadix + o];
+  uint32_t b = dbase;
+#pragma unroll
+  for (uint32_t c = 0; c < (uint32_t)kChains; ++c) b += c < chain ? h[c] : 0u;
+  return b;
+}
+
+#ifndef MGS_PASS_WAVES_PER_EU
+#define MGS_PASS_WAVES_PER_EU (BLOCK / 128)
+#endif
+
 template <int KT, typename V, int BLOCK, int IPT, typename D>
-__global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
+__global__ __launch_bounds__(BLOCK, MGS_PASS_WAVES_PER_EU) void pass_kernel(
     const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
     const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, ChainMap cm,
     const uint64_t* __restrict__ red, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters,
@@ -314,6 +361,16 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
   // the descriptor stores on waves 0..3, so no wavefront waits for its own
   // store to complete before it can use a look-back load.
   constexpr bool kSplitLookback = MGS_SPLIT_LB && W >= 8;
+  // MGS_SPLIT_LB == 2: waves 4..7 own the descriptors.  They publish the
+  // tile aggregate and issue the first look-back window before the digit
+  // scan, and finish the look-back (inclusive descriptor, digit bases) after
+  // the tile has been scattered into LDS, so its latency overlaps the scan
+  // and the scatter.
+  constexpr bool kDeferLB = kSplitLookback && MGS_SPLIT_LB == 2;
+  // MGS_EARLY_PF (with the deferred look-back): the next tile is taken while
+  // the digits are scanned, and each wavefront loads its part of it as soon
+  // as its keys of the current tile are in LDS.
+  constexpr bool kEarlyPF = kDeferLB && MGS_EARLY_PF;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
@@ -321,7 +378,14 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
   // Tiles are taken by one thread of a wavefront that issues no descriptor
   // stores (its atomic then waits for nothing else); its acquisition state:
   const int taker = kSplitLookback ? kRadix : 0;
-  uint32_t my_chain = blockIdx.x % cm.chains, chains_left = cm.chains;
+  const bool xcd_local = MGS_XCD_LOCAL && cm.chains % kXcds == 0 && ld_agent(&red[2]) == (1ull << kXcds) - 1;
+  uint32_t my_chain = blockIdx.x % cm.chains, chains_left = cm.chains, chain_step = 1;
+  if (xcd_local) {
+    const uint32_t g = cm.chains / kXcds;
+    my_chain = xcc_id() + kXcds * ((blockIdx.x / kXcds) % g);
+    chains_left = g;
+    chain_step = kXcds;
+  }
 
   // Digit base of this byte (keys with a smaller digit), held by the thread
   // of each digit; the base of a chain adds the keys of earlier chains with
@@ -341,7 +405,7 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
 #ifdef MGS_RANK_PEERS
     for (int i = tid; i < W * kRadix; i += BLOCK) (&peers[0][0])[i] = 0;
 #endif
-    if (tid == taker) s_tile = take_tile(cnt, cm, my_chain, chains_left);
+    if (tid == taker) s_tile = take_tile(cnt, cm, my_chain, chains_left, chain_step);
     __syncthreads();
   }
   uint32_t tile = s_tile;
@@ -401,24 +465,32 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
     uint32_t total = 0;
     D* my_desc = desc + (size_t)tile * kRadix + tid;
     const D* chain_desc = desc + (size_t)first * kRadix;
-    D win[kWindow];
-    if (tid < kRadix) {
+    D win[kPassWindow];
+    uint32_t taken = kNoTile;
+    if (kDeferLB && tid >= kRadix && tid < 2 * kRadix) {
+      const uint32_t d = (uint32_t)tid - kRadix;
+      if (kEarlyPF && tid == taker) taken = take_tile(cnt, cm, my_chain, chains_left, chain_step);
+#pragma unroll
+      for (int i = 0; i < W; ++i) total += off[i][d];
+      publish(desc + (size_t)tile * kRadix + d, Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)total, parity), xcd_local);
+      if (lt > 0) lookback_load(chain_desc, (int64_t)lt - 1, d, win);
+    } else if (tid < kRadix) {
 #pragma unroll
       for (int i = 0; i < W; ++i) total += off[i][tid];
       if (!kSplitLookback && lt > 0) lookback_load(chain_desc, (int64_t)lt - 1, (uint32_t)tid, win);
-      st_agent(my_desc, Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)total, parity));
+      if (!kDeferLB) publish(my_desc, Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)total, parity), xcd_local);
     } else if (kSplitLookback && tid < 2 * kRadix) {
       const uint32_t d = (uint32_t)tid - kRadix;
 #ifdef MGS_PHASE_PROBE
       uint32_t st = 0;
       s_prefix[d] = lt > 0 ? lookback(chain_desc, lt, d, parity, &st) : 0u;
       if (d == 0 && g_phase_probe && (tile & 63) == 0)
-        g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 8 + 7] = st;
+        g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 16 + 7] = st;
 #else
       s_prefix[d] = lt > 0 ? lookback(chain_desc, lt, d, parity) : 0u;
 #endif
     }
-    const uint32_t excl = block_excl_scan_256(total, sums, tid);
+    const uint32_t excl = block_excl_scan_256(tid < kRadix ? total : 0u, sums, tid);
     if (tid < kRadix) {
       uint32_t run = excl;
 #pragma unroll
@@ -427,55 +499,98 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void pass_kernel(
         off[i][tid] = run;
         run += c;
       }
-      uint32_t prefix = 0;
-      if (lt > 0) {
-        prefix = kSplitLookback ? s_prefix[tid] : lookback_from(chain_desc, (int64_t)lt - 1, (uint32_t)tid, parity, win);
-        st_agent(my_desc, Desc<D>::make(kStInclusive, (D)(prefix + total), parity));
+      if (!kDeferLB) {
+        uint32_t prefix = 0;
+        if (lt > 0) {
+          prefix = kSplitLookback ? s_prefix[tid] : lookback_from(chain_desc, (int64_t)lt - 1, (uint32_t)tid, parity, win);
+          publish(my_desc, Desc<D>::make(kStInclusive, (D)(prefix + total), parity), xcd_local);
+        }
+        if (chain != cur_chain) {
+          cur_chain = chain;
+          cur_base = chain_digit_base(hp, dbase, chain, tid);
+        }
+        gbase[tid] = cur_base + prefix - excl;
+      } else {
+        if (chain != cur_chain) {
+          cur_chain = chain;
+          cur_base = chain_digit_base(hp, dbase, chain, tid);
+        }
+        s_prefix[tid] = cur_base - excl;  // the digit base without the look-back prefix
       }
-      if (chain != cur_chain) {
-        cur_chain = chain;
-        cur_base = dbase;
-#pragma unroll
-        for (uint32_t c = 0; c < (uint32_t)kChains; ++c)
-          if (c < chain) cur_base += hp[c * kRadix + tid];
-      }
-      gbase[tid] = cur_base + prefix - excl;
       if (tid == 0) PHASE(4);
     }
+    if (kEarlyPF && tid == taker) s_tile = taken;
     __syncthreads();
     PHASE(3);
     // This tile's inclusive prefixes are published: only now take the next
     // tile, so a tile's predecessor is always one that is already being
     // ranked (taking it earlier makes look-backs wait on tiles nobody works
     // on yet).
-    if (tid == taker) {
-      s_tile = take_tile(cnt, cm, my_chain, chains_left);
+    if (!kEarlyPF && tid == taker) {
+      s_tile = take_tile(cnt, cm, my_chain, chains_left, chain_step);
 #ifdef MGS_PHASE_PROBE
       if (g_phase_probe && (tile & 63) == 0)
-        g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 8] = __builtin_amdgcn_s_memrealtime();
+        g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 16] = __builtin_amdgcn_s_memrealtime();
 #endif
     }
 
+    if (full) {
+      // all offset reads first, then all writes (branch-free; the clamp only
+      // keeps a corrupted offset inside the staging buffer)
+      uint32_t pos[IPT];
 #pragma unroll
-    for (int i = 0; i < IPT; ++i) {
-      const uint32_t pos = off[w][pk[i] >> 16] + (pk[i] & 0xFFFFu);
-      if ((full || wb + i * kWave < n) && pos < (uint32_t)TILE) {
-        keys[pos] = kc[i];
-        if constexpr (HV) vstage[pos] = vc[i];
+      for (int i = 0; i < IPT; ++i) pos[i] = min(off[w][pk[i] >> 16] + (pk[i] & 0xFFFFu), (uint32_t)TILE - 1);
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        keys[pos[i]] = kc[i];
+        if constexpr (HV) vstage[pos[i]] = vc[i];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        const uint32_t pos = off[w][pk[i] >> 16] + (pk[i] & 0xFFFFu);
+        if (wb + i * kWave < n && pos < (uint32_t)TILE) {
+          keys[pos] = kc[i];
+          if constexpr (HV) vstage[pos] = vc[i];
+        }
       }
     }
-    __syncthreads();
-    const uint32_t nxt = s_tile;
-
-    // prefetch the next tile while this one is written out; the rank
-    // counters are free again and are cleared for it
+    PHASE(8);
     Bits kn[IPT];
     VS vn[IPT];
-    if (nxt != kNoTile) {
-      const uint32_t tb = nxt * (uint32_t)TILE;
-      load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kn, vn);
+    uint32_t nxt = kNoTile;
+    if (kEarlyPF) {
+      // this wavefront's rank counters and keys are free: clear the counters
+      // and load its part of the next tile
+      nxt = s_tile;
+      if (nxt != kNoTile) {
+        const uint32_t tb = nxt * (uint32_t)TILE;
+        load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kn, vn);
+      }
+      for (int i = lane; i < kRadix; i += kWave) off[w][i] = 0;
     }
-    for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
+    if (kDeferLB && tid >= kRadix && tid < 2 * kRadix && lt > 0) {
+      // this wavefront's part of the tile is in LDS: finish the look-back
+      const uint32_t d = (uint32_t)tid - kRadix;
+      const uint32_t prefix = lookback_from(chain_desc, (int64_t)lt - 1, d, parity, win);
+      publish(desc + (size_t)tile * kRadix + d, Desc<D>::make(kStInclusive, (D)(prefix + total), parity), xcd_local);
+      gbase[d] = s_prefix[d] + prefix;
+    } else if (kDeferLB && tid >= kRadix && tid < 2 * kRadix) {
+      gbase[tid - kRadix] = s_prefix[tid - kRadix];
+    }
+    __syncthreads();
+    PHASE(9);
+    if (!kEarlyPF) {
+      // prefetch the next tile while this one is written out; the rank
+      // counters are free again and are cleared for it
+      nxt = s_tile;
+      if (nxt != kNoTile) {
+        const uint32_t tb = nxt * (uint32_t)TILE;
+        load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kn, vn);
+      }
+      for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
+    }
+    PHASE(10);
 
     // Write-out: consecutive threads store consecutive positions of each
     // digit run.  o < n for every key of a well-formed pass; the clamp only
@@ -704,10 +819,7 @@ __global__ __launch_bounds__(BLOCK, BLOCK / 128) void chunk_pass_kernel(
       st_agent(my_desc, Desc<D>::make(lc == 0 ? kStInclusive : kStAggregate, (D)ctot, parity));
       if (chain != cur_chain) {
         cur_chain = chain;
-        cur_base = dbase;
-#pragma unroll
-        for (uint32_t c = 0; c < (uint32_t)kChains; ++c)
-          if (c < chain) cur_base += hp[c * kRadix + tid];
+        cur_base = chain_digit_base(hp, dbase, chain, tid);
       }
     } else if (tid < 2 * kRadix) {
       if (lc > 0) lookback_load(chain_desc, (int64_t)lc - 1, (uint32_t)(tid - kRadix), win);
@@ -895,7 +1007,7 @@ SortLayout sort_layout(size_t n) {
   L.alt_k = take(n * KB);
   L.alt_v = take(n * VB);
   L.hist = take((size_t)KB * kChains * kRadix * 4);
-  L.red = take(2 * sizeof(uint64_t));
+  L.red = take(3 * sizeof(uint64_t));
   L.counters = take((size_t)KB * kChains * 4);
   L.desc = take((size_t)L.num_tiles * kRadix * (L.wide_desc ? 8 : 4));
   L.total = o;

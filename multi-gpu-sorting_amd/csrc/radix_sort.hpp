@@ -149,7 +149,19 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 // sequence in radix_sort.hip):
 //   MGS_UNI        uniform-digit fast path in ranking and counting
 //   MGS_BATCH_WO   write-out of full tiles in batches of 8 items
-//   MGS_SPLIT_LB   look-back loads on waves 4..7, descriptor stores on 0..3
+//   MGS_SPLIT_LB   1: look-back loads on waves 4..7, descriptor stores on
+//                  0..3; 2: waves 4..7 own the descriptors and finish the
+//                  look-back after the scatter into LDS
+//   MGS_XCD_LOCAL  chains served per XCD with workgroup-scope descriptor
+//                  stores (when the device shows eight XCDs)
+//   MGS_EARLY_PF   (with MGS_SPLIT_LB 2) next tile taken during the digit
+//                  scan and loaded right after each wavefront's scatter
+#ifndef MGS_XCD_LOCAL
+#define MGS_XCD_LOCAL 1
+#endif
+#ifndef MGS_EARLY_PF
+#define MGS_EARLY_PF 1
+#endif
 #ifndef MGS_UNI
 #define MGS_UNI 0
 #endif
@@ -157,7 +169,7 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 #define MGS_BATCH_WO 0
 #endif
 #ifndef MGS_SPLIT_LB
-#define MGS_SPLIT_LB 1
+#define MGS_SPLIT_LB 2
 #endif
 
 // A wavefront whose active lanes all carry one digit (runs of equal bytes in
