@@ -20,7 +20,7 @@ constexpr uint32_t kLookbackSpinLimit = MGS_LB_SPIN_LIMIT;
 #define MGS_LB_WINDOW 8
 #endif
 #ifndef MGS_PASS_LB_WINDOW
-#define MGS_PASS_LB_WINDOW 4
+#define MGS_PASS_LB_WINDOW 2
 #endif
 
 // Wavefront ranking used by every kernel: one LDS atomic per item by
@@ -48,23 +48,48 @@ __device__ __forceinline__ void lookback_load(const D* desc, int64_t t, uint32_t
 // found (or the chain's first tile is passed).  Issuing the first window
 // before the tile publishes its own aggregate keeps that store's completion
 // off the critical path.
+// Sums the leading published descriptors of a window loaded at predecessor
+// t into `prefix`; returns how many were consumed and sets `done` when an
+// inclusive prefix ended the walk.
+template <typename D, int NW>
+__device__ __forceinline__ int lookback_consume(int64_t t, int parity, const D (&v)[NW], uint32_t& prefix,
+                                                bool& done) {
+  int consumed = 0;
+  done = false;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) {
+    if (done || consumed != j || t - j < 0) continue;
+    const uint32_t st = Desc<D>::status(v[j], parity);
+    if (st == kStEmpty) continue;
+    prefix += (uint32_t)Desc<D>::value(v[j]);
+    consumed = j + 1;
+    if (st == kStInclusive) done = true;
+  }
+  return consumed;
+}
+
+// One look-back round that does not wait for the next one: consumes the
+// window loaded at predecessor t and, unless the walk is over, moves t back
+// and issues the loads of the next window.  Returns true when the prefix is
+// complete.
+template <typename D, int NW>
+__device__ __forceinline__ bool lookback_step(const D* desc, int64_t& t, uint32_t d, int parity, D (&v)[NW],
+                                              uint32_t& prefix) {
+  bool done;
+  const int consumed = lookback_consume(t, parity, v, prefix, done);
+  if (done || t - consumed < 0) return true;
+  t -= consumed;
+  lookback_load(desc, t, d, v);
+  return false;
+}
+
 template <typename D, int NW>
 __device__ __forceinline__ uint32_t lookback_from(const D* desc, int64_t t, uint32_t d, int parity,
-                                                  D (&v)[NW], uint32_t* stats = nullptr) {
-  uint32_t prefix = 0;
+                                                  D (&v)[NW], uint32_t* stats = nullptr, uint32_t prefix = 0) {
   uint32_t spins = 0, rounds = 1;
   for (;;) {
-    int consumed = 0;
-    bool done = false;
-#pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      if (done || consumed != j || t - j < 0) continue;
-      const uint32_t st = Desc<D>::status(v[j], parity);
-      if (st == kStEmpty) continue;
-      prefix += (uint32_t)Desc<D>::value(v[j]);
-      consumed = j + 1;
-      if (st == kStInclusive) done = true;
-    }
+    bool done;
+    const int consumed = lookback_consume(t, parity, v, prefix, done);
     if (done || t - consumed < 0) break;
     if (consumed == 0) {
       // Bounded wait: a predecessor tile always publishes within microseconds;

@@ -371,6 +371,9 @@ __global__ __launch_bounds__(BLOCK, MGS_PASS_WAVES_PER_EU) void pass_kernel(
   // the digits are scanned, and each wavefront loads its part of it as soon
   // as its keys of the current tile are in LDS.
   constexpr bool kEarlyPF = kDeferLB && MGS_EARLY_PF;
+  // MGS_LB_STEP (with the deferred look-back): one look-back round is taken
+  // before the scatter, so the next round trip overlaps the scatter.
+  constexpr bool kLbStep = kDeferLB && MGS_LB_STEP;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
@@ -533,6 +536,11 @@ __global__ __launch_bounds__(BLOCK, MGS_PASS_WAVES_PER_EU) void pass_kernel(
         g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 16] = __builtin_amdgcn_s_memrealtime();
 #endif
     }
+    int64_t lb_t = (int64_t)lt - 1;
+    uint32_t lb_prefix = 0;
+    bool lb_done = false;
+    if (kLbStep && tid >= kRadix && tid < 2 * kRadix && lt > 0)
+      lb_done = lookback_step(chain_desc, lb_t, (uint32_t)tid - kRadix, parity, win, lb_prefix);
 
     if (full) {
       // all offset reads first, then all writes (branch-free; the clamp only
@@ -559,11 +567,17 @@ __global__ __launch_bounds__(BLOCK, MGS_PASS_WAVES_PER_EU) void pass_kernel(
     Bits kn[IPT];
     VS vn[IPT];
     uint32_t nxt = kNoTile;
+    // The look-back wavefronts (4..7) load their part of the next tile only
+    // after the look-back when MGS_EARLY_PF is 2: loads complete in issue
+    // order, so a look-back round issued after the tile loads would wait for
+    // them.
+    const bool lb_wave = kDeferLB && tid >= kRadix && tid < 2 * kRadix;
+    const bool pf_late = MGS_EARLY_PF == 2 && lb_wave && lt > 0;
     if (kEarlyPF) {
       // this wavefront's rank counters and keys are free: clear the counters
       // and load its part of the next tile
       nxt = s_tile;
-      if (nxt != kNoTile) {
+      if (nxt != kNoTile && !pf_late) {
         const uint32_t tb = nxt * (uint32_t)TILE;
         load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kn, vn);
       }
@@ -572,11 +586,22 @@ __global__ __launch_bounds__(BLOCK, MGS_PASS_WAVES_PER_EU) void pass_kernel(
     if (kDeferLB && tid >= kRadix && tid < 2 * kRadix && lt > 0) {
       // this wavefront's part of the tile is in LDS: finish the look-back
       const uint32_t d = (uint32_t)tid - kRadix;
-      const uint32_t prefix = lookback_from(chain_desc, (int64_t)lt - 1, d, parity, win);
+#ifdef MGS_PHASE_PROBE
+      uint32_t st = 0;
+      const uint32_t prefix = lb_done ? lb_prefix : lookback_from(chain_desc, lb_t, d, parity, win, &st, lb_prefix);
+      if (d == 0 && g_phase_probe && (tile & 63) == 0)
+        g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 16 + 7] = st + (kLbStep ? 1 : 0);
+#else
+      const uint32_t prefix = lb_done ? lb_prefix : lookback_from(chain_desc, lb_t, d, parity, win, nullptr, lb_prefix);
+#endif
       publish(desc + (size_t)tile * kRadix + d, Desc<D>::make(kStInclusive, (D)(prefix + total), parity), xcd_local);
       gbase[d] = s_prefix[d] + prefix;
     } else if (kDeferLB && tid >= kRadix && tid < 2 * kRadix) {
       gbase[tid - kRadix] = s_prefix[tid - kRadix];
+    }
+    if (kEarlyPF && pf_late && nxt != kNoTile) {
+      const uint32_t tb = nxt * (uint32_t)TILE;
+      load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kn, vn);
     }
     __syncthreads();
     PHASE(9);

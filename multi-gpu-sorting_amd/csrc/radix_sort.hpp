@@ -86,14 +86,24 @@ enum : uint32_t { kStEmpty = 0, kStAggregate = 1, kStInclusive = 2 };
 #ifndef MGS_OS_BLOCK
 #define MGS_OS_BLOCK 512
 #endif
+// Items per thread for 4-byte keys alone, for 5..12 bytes of key and
+// payload, and for more.  18 x 512 keys is the largest u32 tile whose
+// registers fit two workgroups per CU without spilling in the hot loop.
 #ifndef MGS_OS_IPT
-#define MGS_OS_IPT 16
+#define MGS_OS_IPT 18
+#endif
+#ifndef MGS_OS_IPT_MID
+#define MGS_OS_IPT_MID 8
+#endif
+#ifndef MGS_OS_IPT_WIDE
+#define MGS_OS_IPT_WIDE 4
 #endif
 // A tile stages at most 48 KiB of keys and payloads in LDS, so that two
 // workgroups fit on one CU beside their counters.
-template <int KB, int VB> struct TileCfg {
+template <int KB, int VB, int IPT4 = MGS_OS_IPT, int IPT12 = MGS_OS_IPT_MID, int IPTW = MGS_OS_IPT_WIDE>
+struct TileCfg {
   static constexpr int kBlock = MGS_OS_BLOCK;
-  static constexpr int kIpt = (KB + VB) <= 4 ? MGS_OS_IPT : ((KB + VB) <= 12 ? MGS_OS_IPT / 2 : MGS_OS_IPT / 4);
+  static constexpr int kIpt = (KB + VB) <= 4 ? IPT4 : ((KB + VB) <= 12 ? IPT12 : IPTW);
   static constexpr int kTile = kBlock * kIpt;
 };
 
@@ -154,8 +164,13 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 //                  look-back after the scatter into LDS
 //   MGS_XCD_LOCAL  chains served per XCD with workgroup-scope descriptor
 //                  stores (when the device shows eight XCDs)
+//   MGS_LB_STEP    (with MGS_SPLIT_LB 2) one look-back round taken before
+//                  the scatter
 //   MGS_EARLY_PF   (with MGS_SPLIT_LB 2) next tile taken during the digit
 //                  scan and loaded right after each wavefront's scatter
+#ifndef MGS_LB_STEP
+#define MGS_LB_STEP 1
+#endif
 #ifndef MGS_XCD_LOCAL
 #define MGS_XCD_LOCAL 1
 #endif

@@ -516,7 +516,9 @@ struct SegLayout {
 
 template <int KB, int VB>
 struct SegCfg {
-  using T = TileCfg<KB, VB>;
+  // the segmented passes keep their look-back before the scatter and a wider
+  // window, so they use 16 items per thread for 4-byte keys
+  using T = TileCfg<KB, VB, 16, 8, 4>;
   using S = SmallCfg<KB, VB>;
   static constexpr int kSmallMax = S::kTile;
   static constexpr int kMediumMax = 1 << 17;
