@@ -28,6 +28,7 @@ struct SortServer::Batch {
   uint64_t* d_bounds = nullptr;
   int ring = -1;
   size_t keys = 0;
+  bool solo = false;  // one request, sorted with mgs_sort
   hipEvent_t ev_in = nullptr, ev_sort = nullptr, ev_out = nullptr;
 };
 
@@ -109,10 +110,14 @@ std::unique_ptr<SortServer::Batch> SortServer::FormBatch(std::unique_lock<std::m
   SortRequest* front = queue_.front();
   b->kt = front->kt;
   b->vt = front->vt;
+  auto large = [&](const SortRequest* r) { return cfg_.solo_min_keys > 0 && r->n >= cfg_.solo_min_keys; };
+  b->solo = !cfg_.batched || large(front);
   std::vector<size_t> taken;  // queue positions
   for (size_t i = 0; i < queue_.size() && b->reqs.size() < cfg_.max_batch_requests; ++i) {
     SortRequest* r = queue_[i];
     if (r->kt != b->kt || r->vt != b->vt) continue;
+    if (!b->reqs.empty() && b->solo) break;
+    if (!b->reqs.empty() && large(r)) continue;  // sorted alone when it reaches the front
     if (!b->reqs.empty() && b->keys + r->n > cfg_.max_batch_keys) break;
     char* slot = static_cast<char*>(r->slot);
     bool own = false;
@@ -139,8 +144,8 @@ std::unique_ptr<SortServer::Batch> SortServer::FormBatch(std::unique_lock<std::m
   };
   while (!b->reqs.empty()) {
     const size_t nseg = b->reqs.size();
-    b->temp_bytes = cfg_.batched ? mgs_segmented_sort_temp_bytes(b->kt, b->vt, b->keys, nseg)
-                                 : mgs_sort_temp_bytes(b->kt, b->vt, b->keys);
+    b->temp_bytes = b->solo ? mgs_sort_temp_bytes(b->kt, b->vt, b->keys)
+                            : mgs_segmented_sort_temp_bytes(b->kt, b->vt, b->keys, nseg);
     b->scratch = pool_->Allocate(AlignUp(b->temp_bytes) + 3 * nseg * sizeof(uint64_t));
     if (b->scratch != nullptr) break;
     release_from(nseg / 2);  // smaller batch, smaller scratch
@@ -186,7 +191,7 @@ void SortServer::Launch(Batch& b) {
   HIP_CHECK(hipStreamWaitEvent(s_sort_, b.ev_in, 0));
   char* base = pool_->base();
   int st = MGS_OK;
-  if (cfg_.batched) {
+  if (!b.solo) {
     uint64_t* h = h_bounds_[b.ring];
     for (size_t i = 0; i < nseg; ++i) {
       const uint64_t kb = (uint64_t)(b.slots[i] - base) / K;

@@ -10,6 +10,10 @@
 //   overlap     up to max_inflight batches are on the device at once: host to
 //               device copies of batch b+1, the sort of batch b and the copies
 //               back of batch b-1 run on three streams ordered by events
+//   large       in batched mode a request of at least solo_min_keys keys is
+//               sorted alone with mgs_sort: the plain sort already runs at
+//               its per-type rate there, and a batch holding it would make
+//               the small requests wait for it
 //   plain mode  the same loop with one request per launch sequence (mgs_sort)
 //
 // Requests either bring host arrays (host_data: copied in on admission and
@@ -58,6 +62,7 @@ struct ServerConfig {
   int max_inflight = 3;
   size_t max_batch_keys = size_t(1) << 30;
   size_t max_batch_requests = 4096;
+  size_t solo_min_keys = size_t(1) << 21;  // 0: batch every request
 };
 
 struct ServerStats {
