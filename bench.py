@@ -249,6 +249,18 @@ def main():
     if not ok:
         raise RuntimeError("sorted output failed the check")
 
+    # ceiling reference: a plain device-to-device copy of the same 4 GiB
+    # (HIP's copy kernel), which reads and writes the bytes a pass moves
+    copy_ms = []
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        copy_ms.append(e0.elapsed_time(e1))
+    copy_gbps = n * PASS_BYTES_PER_KEY / (min(copy_ms) / 1e3) / 1e9
+
     pass_ms = [stage_ms[f"pass{p}"] for p in range(KEY_BYTES)]
     avg_pass_ms = sum(pass_ms) / len(pass_ms)
     achieved = n * PASS_BYTES_PER_KEY / (avg_pass_ms / 1e3) / 1e9
@@ -275,6 +287,9 @@ def main():
                      "frac": round(achieved / PEAK_GBPS, 4), "traffic": traffic_from_profiles(),
                      "alg_bytes_per_launch": n * PASS_BYTES_PER_KEY, "avg_launch_ms": round(avg_pass_ms, 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "copy_ceiling": {"what": "device-to-device copy of the 4 GiB key array (hipMemcpy's kernel), same bytes as one pass",
+                         "gbps": round(copy_gbps, 1), "frac_of_peak": round(copy_gbps / PEAK_GBPS, 4),
+                         "pass_vs_copy": round(achieved / copy_gbps, 4)},
     }
     del src, out, temp
     torch.cuda.empty_cache()
