@@ -1,5 +1,6 @@
 // Onesweep LSD radix sort kernels and their launch sequence (see
 // radix_sort.hpp for the algorithm outline).
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -77,7 +78,8 @@ __device__ __forceinline__ void block_chain(uint32_t b, uint32_t grid, uint32_t 
 template <int KT, int BLOCK, bool VEC>
 __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>::Bits* __restrict__ keys, uint32_t n,
                                                         uint32_t* __restrict__ hist, uint64_t* __restrict__ red,
-                                                        uint32_t* __restrict__ desc, size_t desc_words, ChainMap cm) {
+                                                        uint32_t* __restrict__ desc, size_t desc_words, ChainMap cm,
+                                                        int record_xcd) {
   using Bits = typename Codec<KT>::Bits;
   __shared__ uint32_t h[kRadix];
   __shared__ Bits s_or[BLOCK / kWave], s_and[BLOCK / kWave];
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>
     atomicOr(reinterpret_cast<unsigned long long*>(&red[0]), (unsigned long long)a);
     atomicAnd(reinterpret_cast<unsigned long long*>(&red[1]),
               (unsigned long long)b | (sizeof(Bits) == 4 ? 0xFFFFFFFF00000000ull : 0ull));
-    atomicOr(reinterpret_cast<unsigned long long*>(&red[2]), 1ull << xcc_id());
+    if (record_xcd) atomicOr(reinterpret_cast<unsigned long long*>(&red[2]), 1ull << xcc_id());
   }
   for (int i = tid; i < kRadix; i += BLOCK) {
     const uint32_t v = h[i];
@@ -1002,6 +1004,14 @@ namespace {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
+// XCD-local chains can be switched off at run time with MGS_XCD_LOCAL=0 in
+// the environment (read on every call): the upfront kernel then records no
+// XCDs and every pass uses agent-scope descriptors.
+int xcd_local_allowed() {
+  const char* e = std::getenv("MGS_XCD_LOCAL");
+  return (e != nullptr && e[0] == '0' && e[1] == 0) ? 0 : 1;
+}
+
 struct SortLayout {
   size_t alt_k, alt_v, hist, red, counters, desc, total;
   uint32_t num_tiles;
@@ -1123,9 +1133,11 @@ int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t 
         std::max<size_t>((cm.chain_keys + 16 * HB - 1) / (16 * HB), 1), std::max<uint32_t>(2048 / cm.chains, 1));
     const unsigned grid = per_chain * cm.chains;
     if ((reinterpret_cast<uintptr_t>(kin) % 16) == 0)
-      upfront_kernel<KT, HB, true><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words, cm);
+      upfront_kernel<KT, HB, true><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words, cm,
+                                                       xcd_local_allowed());
     else
-      upfront_kernel<KT, HB, false><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words, cm);
+      upfront_kernel<KT, HB, false><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words, cm,
+                                                        xcd_local_allowed());
   }
   auto launch_pass = [&](auto kernel, auto* dsc, int p) {
     const uint32_t res = resident_blocks(kernel, Cfg::kBlock, 0);

@@ -123,3 +123,27 @@ def test_empty_is_noop(mg):
     import torch
     dk = torch.zeros(0, dtype=torch.int32, device="cuda")
     mg.sort("u32", dk)
+
+
+@pytest.mark.parametrize("kt,vdt", [(O.K_U32, None), (O.K_U64, np.uint32), (O.K_F32, np.uint32)],
+                         ids=["u32", "u64+v32", "f32+v32"])
+def test_xcd_local_switch_same_result(mg, kt, vdt, monkeypatch):
+    """Both descriptor modes of the LSD passes give the oracle's result on one
+    temp buffer used alternately: XCD-local chains (workgroup-scope stores,
+    chains served per XCD, the default) and MGS_XCD_LOCAL=0 (agent scope)."""
+    import torch
+    rng = np.random.default_rng(31 + kt)
+    n = 3_000_017  # enough tiles for 16 chains
+    keys = random_keys(kt, n, rng, "few")
+    vals = None if vdt is None else random_vals(vdt, n, rng)
+    ek, ev = O.sort(kt, keys, vals)
+    vt = mg.VAL_NONE if vdt is None else mg.VAL_U32
+    temp = mg.temp_buffer(mg.lib.mgs_sort_temp_bytes(kt, vt, n), "cuda")
+    for mode in ("0", "1", "0", "1"):
+        monkeypatch.setenv("MGS_XCD_LOCAL", mode)
+        dk, dv = to_dev(keys), to_dev(vals)
+        mg.sort(KEY_NAME[kt], dk, dv, temp=temp)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(dk, keys.dtype), ek), mode
+        if vdt is not None:
+            assert np.array_equal(from_dev(dv, vals.dtype), ev), mode
