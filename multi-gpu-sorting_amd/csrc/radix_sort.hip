@@ -284,8 +284,6 @@ __device__ uint64_t* g_phase_probe;
 // One onesweep pass over byte `pass`, persistent workgroups, chained tiles.
 // hist is [P][kChains][256]; counters is [P][kChains].
 // ---------------------------------------------------------------------------
-// Wavefronts per SIMD the pass kernel is compiled for (caps its VGPRs):
-// two 512-thread workgroups per CU by default.
 // Digit base of a chain: `dbase` (keys with a smaller digit) plus the keys of
 // earlier chains with the same digit.  The row offset is made opaque so the
 // compiler does not hold the per-chain addresses in registers across the
@@ -304,12 +302,8 @@ __device__ __forceinline__ uint32_t chain_digit_base(const uint32_t* __restrict_
   return b;
 }
 
-#ifndef MGS_PASS_WAVES_PER_EU
-#define MGS_PASS_WAVES_PER_EU (BLOCK / 128)
-#endif
-
-template <int KT, typename V, int BLOCK, int IPT, typename D>
-__global__ __launch_bounds__(BLOCK, MGS_PASS_WAVES_PER_EU) void pass_kernel(
+template <int KT, typename V, int BLOCK, int IPT, typename D, int WAVES>
+__global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
     const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, ChainMap cm,
     const uint64_t* __restrict__ red, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters,
@@ -1020,16 +1014,16 @@ struct SortLayout {
 
 // Keys per look-back unit of a pass: a chunk of MGS_CHUNK_TILES tiles when
 // the tile has the eight wavefronts the chunked kernel needs, else a tile.
-template <int KB, int VB>
+template <int KB, int VB, bool BIG>
 struct PassUnit {
-  using Cfg = TileCfg<KB, VB>;
+  using Cfg = PassCfg<KB, VB, BIG>;
   static constexpr int kChunkTiles = (MGS_CHUNK_TILES > 1 && Cfg::kBlock >= 512) ? MGS_CHUNK_TILES : 1;
   static constexpr size_t kKeys = (size_t)Cfg::kTile * kChunkTiles;
 };
 
-template <int KB, int VB>
-SortLayout sort_layout(size_t n) {
-  constexpr size_t unit = PassUnit<KB, VB>::kKeys;
+template <int KB, int VB, bool BIG>
+SortLayout sort_layout_cfg(size_t n) {
+  constexpr size_t unit = PassUnit<KB, VB, BIG>::kKeys;
   SortLayout L{};
   L.num_tiles = (uint32_t)((n + unit - 1) / unit);
   L.wide_desc = n > (size_t(1) << 30);
@@ -1047,6 +1041,11 @@ SortLayout sort_layout(size_t n) {
   L.desc = take((size_t)L.num_tiles * kRadix * (L.wide_desc ? 8 : 4));
   L.total = o;
   return L;
+}
+
+template <int KB, int VB>
+SortLayout sort_layout(size_t n) {
+  return use_big_tiles(KB, VB, n) ? sort_layout_cfg<KB, VB, true>(n) : sort_layout_cfg<KB, VB, false>(n);
 }
 
 // Workgroups of `kernel` that fit on the current device at once (persistent
@@ -1084,15 +1083,15 @@ struct Marks {
   }
 };
 
-template <int KT, int VT>
-int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t n, void* temp, size_t temp_bytes,
-               hipStream_t s, Marks& mark) {
+template <int KT, int VT, bool BIG>
+int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, size_t n, void* temp, size_t temp_bytes,
+                   hipStream_t s, Marks& mark) {
   using Bits = typename Codec<KT>::Bits;
   using V = typename ValOf<VT>::T;
   constexpr int KB = sizeof(Bits);
   constexpr int VB = kValBytes<V>;
   constexpr int P = KB;
-  using Cfg = TileCfg<KB, VB>;
+  using Cfg = PassCfg<KB, VB, BIG>;
   using SCfg = SmallCfg<KB, VB>;
   using VP = typename std::conditional<kHasVal<V>, V, uint32_t>::type;
 
@@ -1108,7 +1107,7 @@ int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t 
     return check_launch();
   }
 
-  const SortLayout L = sort_layout<KB, VB>(n);
+  const SortLayout L = sort_layout_cfg<KB, VB, BIG>(n);
   if (temp == nullptr || temp_bytes < L.total) return MGS_ERR_TEMP_TOO_SMALL;
   char* t = static_cast<char*>(temp);
   Bits* ka = reinterpret_cast<Bits*>(t + L.alt_k);
@@ -1120,7 +1119,7 @@ int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t 
   const size_t desc_words = (size_t)L.num_tiles * kRadix * (L.wide_desc ? 2 : 1);
   const int aliased = (kin == kout) || (kHasVal<V> && vin == vout);
 
-  using PU = PassUnit<KB, VB>;
+  using PU = PassUnit<KB, VB, BIG>;
   const ChainMap cm = make_chain_map(n, (uint32_t)PU::kKeys, kChains);
 
   mark(s);
@@ -1155,9 +1154,9 @@ int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t 
         launch_pass(chunk_pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, PU::kChunkTiles, uint32_t>, (uint32_t*)desc, p);
     } else {
       if (L.wide_desc)
-        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint64_t>, (uint64_t*)desc, p);
+        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint64_t, Cfg::kWavesPerEu>, (uint64_t*)desc, p);
       else
-        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint32_t>, (uint32_t*)desc, p);
+        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint32_t, Cfg::kWavesPerEu>, (uint32_t*)desc, p);
     }
   }
   mark(s);
@@ -1167,6 +1166,19 @@ int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t 
   }
   mark(s);
   return check_launch();
+}
+
+template <int KT, int VT>
+int sort_typed(const void* kin, void* kout, const void* vin, void* vout, size_t n, void* temp, size_t temp_bytes,
+               hipStream_t s, Marks& mark) {
+  constexpr int KB = (int)sizeof(typename Codec<KT>::Bits);
+  constexpr int VB = kValBytes<typename ValOf<VT>::T>;
+  if constexpr (KB + VB <= 4)
+    return sort_typed_cfg<KT, VT, true>(kin, kout, vin, vout, n, temp, temp_bytes, s, mark);
+  else if (use_big_tiles(KB, VB, n))
+    return sort_typed_cfg<KT, VT, true>(kin, kout, vin, vout, n, temp, temp_bytes, s, mark);
+  else
+    return sort_typed_cfg<KT, VT, false>(kin, kout, vin, vout, n, temp, temp_bytes, s, mark);
 }
 
 template <int KT>

@@ -86,26 +86,55 @@ enum : uint32_t { kStEmpty = 0, kStAggregate = 1, kStInclusive = 2 };
 #ifndef MGS_OS_BLOCK
 #define MGS_OS_BLOCK 512
 #endif
-// Items per thread for 4-byte keys alone, for 5..12 bytes of key and
-// payload, and for more.  18 x 512 keys is the largest u32 tile whose
-// registers fit two workgroups per CU without spilling in the hot loop.
+// Items per thread for 4-byte keys alone, for 8 bytes of key and payload,
+// for 12, and for more.  18 x 512 keys is the largest u32 tile whose
+// registers fit two workgroups per CU without spilling in the hot loop; the
+// wider records run one workgroup per CU with the largest tile its LDS holds
+// beside the counters.
 #ifndef MGS_OS_IPT
 #define MGS_OS_IPT 18
 #endif
+#ifndef MGS_OS_IPT_8B
+#define MGS_OS_IPT_8B 28
+#endif
 #ifndef MGS_OS_IPT_MID
-#define MGS_OS_IPT_MID 8
+#define MGS_OS_IPT_MID 16
 #endif
 #ifndef MGS_OS_IPT_WIDE
-#define MGS_OS_IPT_WIDE 4
+#define MGS_OS_IPT_WIDE 12
 #endif
 // A tile stages at most 48 KiB of keys and payloads in LDS, so that two
 // workgroups fit on one CU beside their counters.
-template <int KB, int VB, int IPT4 = MGS_OS_IPT, int IPT12 = MGS_OS_IPT_MID, int IPTW = MGS_OS_IPT_WIDE>
+// Wavefronts per SIMD the LSD pass kernel is compiled for, which caps its
+// VGPRs: two 512-thread workgroups per CU for 4-byte keys alone (128
+// VGPRs), one for wider records, whose larger tiles need the registers and
+// the LDS of a whole CU (measured at 2^28-2^29 keys: 16-21% faster for u32
+// and u64 keys with a payload than two workgroups with 8 or 4 items per
+// thread).  Arrays below kBigTileMin keys keep the small tiles and two
+// workgroups per CU: they have too few tiles to fill the device otherwise.
+#ifndef MGS_PASS_WAVES_4B
+#define MGS_PASS_WAVES_4B 4
+#endif
+#ifndef MGS_PASS_WAVES_WIDE
+#define MGS_PASS_WAVES_WIDE 2
+#endif
+template <int KB, int VB, int IPT4 = MGS_OS_IPT, int IPT8 = MGS_OS_IPT_8B, int IPT12 = MGS_OS_IPT_MID,
+          int IPTW = MGS_OS_IPT_WIDE, int WAVES_WIDE = MGS_PASS_WAVES_WIDE>
 struct TileCfg {
   static constexpr int kBlock = MGS_OS_BLOCK;
-  static constexpr int kIpt = (KB + VB) <= 4 ? IPT4 : ((KB + VB) <= 12 ? IPT12 : IPTW);
+  static constexpr int kIpt = (KB + VB) <= 4 ? IPT4 : (KB + VB) <= 8 ? IPT8 : (KB + VB) <= 12 ? IPT12 : IPTW;
   static constexpr int kTile = kBlock * kIpt;
+  static constexpr int kWavesPerEu = (KB + VB) <= 4 ? MGS_PASS_WAVES_4B : WAVES_WIDE;
 };
+
+// Tile geometry of the LSD passes: large tiles, or the small ones below
+// kBigTileMin keys (records wider than 4 bytes only; 4-byte keys have one
+// geometry).
+constexpr size_t kBigTileMin = size_t(1) << 21;
+template <int KB, int VB, bool BIG>
+using PassCfg = typename std::conditional<BIG || (KB + VB) <= 4, TileCfg<KB, VB>,
+                                          TileCfg<KB, VB, MGS_OS_IPT, 8, 8, 4, MGS_PASS_WAVES_4B>>::type;
+inline bool use_big_tiles(int kb, int vb, size_t n) { return kb + vb <= 4 || n >= kBigTileMin; }
 
 // Single-workgroup LDS sort geometry.
 template <int KB, int VB> struct SmallCfg {

@@ -518,7 +518,7 @@ template <int KB, int VB>
 struct SegCfg {
   // the segmented passes keep their look-back before the scatter and a wider
   // window, so they use 16 items per thread for 4-byte keys
-  using T = TileCfg<KB, VB, 16, 8, 4>;
+  using T = TileCfg<KB, VB, 16, 8, 8, 4, 4>;
   using S = SmallCfg<KB, VB>;
   static constexpr int kSmallMax = S::kTile;
   static constexpr int kMediumMax = 1 << 17;

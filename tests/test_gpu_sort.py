@@ -147,3 +147,21 @@ def test_xcd_local_switch_same_result(mg, kt, vdt, monkeypatch):
         assert np.array_equal(from_dev(dk, keys.dtype), ek), mode
         if vdt is not None:
             assert np.array_equal(from_dev(dv, vals.dtype), ev), mode
+
+
+@pytest.mark.parametrize("kt,vdt", [(O.K_U64, None), (O.K_I64, np.uint64), (O.K_U32, np.uint32), (O.K_F64, np.uint32),
+                                    (O.K_I32, np.uint64)],
+                         ids=["u64", "i64+v64", "u32+v32", "f64+v32", "i32+v64"])
+def test_sort_large_tile_geometry(mg, kt, vdt):
+    """Arrays of 2^21 keys and more with records wider than 4 bytes use the
+    large tiles (one workgroup per CU); parity with the oracle, ragged last
+    tile, payloads stable."""
+    rng = np.random.default_rng(77 + kt)
+    n = (1 << 21) + 12_345
+    keys = random_keys(kt, n, rng, "few")
+    vals = None if vdt is None else random_vals(vdt, n, rng)
+    ek, ev = O.sort(kt, keys, vals)
+    rk, rv = run_sort(mg, kt, keys, vals, inplace=False)
+    assert np.array_equal(rk, ek)
+    if vdt is not None:
+        assert np.array_equal(rv, ev)
