@@ -22,9 +22,9 @@ constexpr int kMaxPasses = 8;
 // Buffer ids of the ping-pong plan.
 enum : int32_t { kBufIn = 0, kBufAlt = 1, kBufOut = 2 };
 
-// Look-back chains per pass (two per XCD by default).
+// Look-back chains per pass (one per XCD by default).
 #ifndef MGS_CHAINS
-#define MGS_CHAINS 16
+#define MGS_CHAINS 8
 #endif
 constexpr int kChains = MGS_CHAINS;
 
@@ -87,12 +87,18 @@ enum : uint32_t { kStEmpty = 0, kStAggregate = 1, kStInclusive = 2 };
 #define MGS_OS_BLOCK 512
 #endif
 // Items per thread for 4-byte keys alone, for 8 bytes of key and payload,
-// for 12, and for more.  18 x 512 keys is the largest u32 tile whose
-// registers fit two workgroups per CU without spilling in the hot loop; the
-// wider records run one workgroup per CU with the largest tile its LDS holds
-// beside the counters.
+// for 12, and for more.  18 is the most a u32 thread holds in 128 VGPRs
+// (four wavefronts per SIMD) without spilling in the hot loop; the wider
+// records run 512-thread workgroups, one per CU, with the largest tile the
+// LDS holds beside the counters.
 #ifndef MGS_OS_IPT
 #define MGS_OS_IPT 18
+#endif
+// 4-byte keys alone run one 1024-thread workgroup per CU (16 wavefronts,
+// 18432-key tiles): measured 5% faster at 2^30 keys than two 512-thread
+// workgroups with 9216-key tiles, and 23% faster at 2^22.
+#ifndef MGS_OS_BLOCK_4B
+#define MGS_OS_BLOCK_4B 1024
 #endif
 #ifndef MGS_OS_IPT_8B
 #define MGS_OS_IPT_8B 28
@@ -103,15 +109,14 @@ enum : uint32_t { kStEmpty = 0, kStAggregate = 1, kStInclusive = 2 };
 #ifndef MGS_OS_IPT_WIDE
 #define MGS_OS_IPT_WIDE 12
 #endif
-// A tile stages at most 48 KiB of keys and payloads in LDS, so that two
-// workgroups fit on one CU beside their counters.
 // Wavefronts per SIMD the LSD pass kernel is compiled for, which caps its
-// VGPRs: two 512-thread workgroups per CU for 4-byte keys alone (128
-// VGPRs), one for wider records, whose larger tiles need the registers and
+// VGPRs: four for 4-byte keys alone (one 1024-thread workgroup per CU, 128
+// VGPRs), two for wider records, whose larger tiles need the registers and
 // the LDS of a whole CU (measured at 2^28-2^29 keys: 16-21% faster for u32
-// and u64 keys with a payload than two workgroups with 8 or 4 items per
-// thread).  Arrays below kBigTileMin keys keep the small tiles and two
-// workgroups per CU: they have too few tiles to fill the device otherwise.
+// and u64 keys with a payload than two 512-thread workgroups with 8 or 4
+// items per thread).  Arrays below kBigTileMin keys keep those small tiles
+// and two workgroups per CU: they have too few tiles to fill the device
+// otherwise.
 #ifndef MGS_PASS_WAVES_4B
 #define MGS_PASS_WAVES_4B 4
 #endif
@@ -119,9 +124,9 @@ enum : uint32_t { kStEmpty = 0, kStAggregate = 1, kStInclusive = 2 };
 #define MGS_PASS_WAVES_WIDE 2
 #endif
 template <int KB, int VB, int IPT4 = MGS_OS_IPT, int IPT8 = MGS_OS_IPT_8B, int IPT12 = MGS_OS_IPT_MID,
-          int IPTW = MGS_OS_IPT_WIDE, int WAVES_WIDE = MGS_PASS_WAVES_WIDE>
+          int IPTW = MGS_OS_IPT_WIDE, int WAVES_WIDE = MGS_PASS_WAVES_WIDE, int BLOCK4 = MGS_OS_BLOCK_4B>
 struct TileCfg {
-  static constexpr int kBlock = MGS_OS_BLOCK;
+  static constexpr int kBlock = (KB + VB) <= 4 ? BLOCK4 : MGS_OS_BLOCK;
   static constexpr int kIpt = (KB + VB) <= 4 ? IPT4 : (KB + VB) <= 8 ? IPT8 : (KB + VB) <= 12 ? IPT12 : IPTW;
   static constexpr int kTile = kBlock * kIpt;
   static constexpr int kWavesPerEu = (KB + VB) <= 4 ? MGS_PASS_WAVES_4B : WAVES_WIDE;

@@ -529,9 +529,9 @@ struct SegCfg {
   // tiles of the large segments' passes: 16 items per thread for 4-byte
   // keys (two workgroups per CU); wider records use MGS_SEG_IPT_WIDE items
   // and the LDS of a whole CU, as the plain sort's large tiles do
-  using T = TileCfg<KB, VB, 16, MGS_SEG_IPT_WIDE, MGS_SEG_IPT_WIDE, MGS_SEG_IPT_WIDE16, 4>;
+  using T = TileCfg<KB, VB, 16, MGS_SEG_IPT_WIDE, MGS_SEG_IPT_WIDE, MGS_SEG_IPT_WIDE16, 4, MGS_OS_BLOCK>;
   // one workgroup per medium segment streams it through tiles of this size
-  using M = TileCfg<KB, VB, 16, 8, 8, 4, 4>;
+  using M = TileCfg<KB, VB, 16, 8, 8, 4, 4, MGS_OS_BLOCK>;
   using S = SmallCfg<KB, VB>;
   static constexpr int kSmallMax = S::kTile;
   static constexpr int kMediumMax = 1 << 17;
