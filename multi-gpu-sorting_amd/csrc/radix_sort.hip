@@ -22,16 +22,19 @@ namespace mgs {
 //             look-back descriptors
 //   pass p    p = 0..P-1, persistent workgroups; a trivial byte returns at
 //             once.  The pass's tiles form kChains chains of consecutive
-//             tiles; workgroup b serves chain b mod chains first (the XCD it
-//             runs on, so neighbouring tiles share one L2), then helps the
+//             tiles, each served by the workgroups of one XCD when the
+//             device shows all eight (XCD-local look-back, below), else
+//             workgroup b starts on chain b mod chains and then helps the
 //             others.  Per tile: rank the keys per wavefront (one LDS atomic
-//             each), publish the tile's digit counts, look back over the
-//             chain's predecessors for the digit prefix, sort the tile in LDS
-//             and write each digit run with consecutive lanes on consecutive
-//             addresses.  On the way out every key is counted in the
-//             histogram of the next non-trivial byte, per chain of its output
-//             position -- exactly the per-chain histograms the next pass
-//             starts from.
+//             each); waves 4..7 publish the tile's digit counts and start the
+//             look-back over the chain's predecessors while waves 0..3 scan
+//             the counts; every wavefront scatters its keys into the LDS
+//             tile and loads its part of the next tile; waves 4..7 finish
+//             the look-back; then each digit run is written with consecutive
+//             lanes on consecutive addresses.  On the way out every key is
+//             counted in the histogram of the next non-trivial byte, per
+//             chain of its output position -- exactly the per-chain
+//             histograms the next pass starts from.
 //   fixup     copies the result into keys_out when the pass count left it in
 //             the scratch buffer
 // ---------------------------------------------------------------------------
@@ -521,10 +524,10 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     if (kEarlyPF && tid == taker) s_tile = taken;
     __syncthreads();
     PHASE(3);
-    // This tile's inclusive prefixes are published: only now take the next
-    // tile, so a tile's predecessor is always one that is already being
-    // ranked (taking it earlier makes look-backs wait on tiles nobody works
-    // on yet).
+    // Without the early prefetch the next tile is taken only now, after
+    // this tile's inclusive prefixes are published (with it, the taker took
+    // it during the digit scan, so that its keys can be loaded right after
+    // the scatter).
     if (!kEarlyPF && tid == taker) {
       s_tile = take_tile(cnt, cm, my_chain, chains_left, chain_step);
 #ifdef MGS_PHASE_PROBE
