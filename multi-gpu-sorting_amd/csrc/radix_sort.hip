@@ -305,7 +305,19 @@ __device__ __forceinline__ uint32_t chain_digit_base(const uint32_t* __restrict_
   return b;
 }
 
-template <int KT, typename V, int BLOCK, int IPT, typename D, int WAVES>
+// RANK selects how digits are ranked and counted in LDS:
+//   0  one atomic per key (the only variant launched for smaller sorts);
+//   1  the same, as the first of a pair of launches for one pass: exits at
+//      once when the pass is skewed;
+//   2  uniform-digit ranking (a wavefront whose lanes share one digit takes
+//      one LDS atomic instead of 64 conflicting ones), the second of the
+//      pair: exits at once unless the pass is skewed.
+// A pass is skewed when one digit of its byte holds more than 1/32 of the
+// keys (heavy duplicates).  The choice is made by the pair of launches, not
+// by a branch inside one kernel, because a kernel holding both ranking paths
+// needs more registers than one workgroup of 1024 threads per CU can give
+// and spills.
+template <int KT, typename V, int BLOCK, int IPT, typename D, int WAVES, int RANK = 0>
 __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
     const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, ChainMap cm,
@@ -319,8 +331,10 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
   using VS = typename std::conditional<HV, V, uint32_t>::type;
   static_assert(W >= 4, "need >= 256 threads for the per-digit phase");
 
+  constexpr bool kUni = RANK == 2;
   const uint32_t vary = varying_bytes(red, P);
   if (!(vary & (1u << pass))) {
+    if constexpr (kUni) return;  // the RANK 1 launch of the pair does this
     // Trivial byte.  If it is the last trivial byte below the first varying
     // one, nobody has counted that byte yet: do it here from the input.
     const uint32_t below = vary & ((2u << pass) - 1u);
@@ -393,6 +407,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
   // of each digit; the base of a chain adds the keys of earlier chains with
   // the same digit, recomputed when the workgroup moves to another chain.
   uint32_t dbase = 0, cur_chain = 0xFFFFFFFFu, cur_base = 0;
+  __shared__ uint32_t s_skew;
   {
     uint32_t tot = 0;
     if (tid < kRadix) {
@@ -400,13 +415,25 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
       for (uint32_t c = 0; c < (uint32_t)kChains; ++c)
         if (c < cm.chains) tot += hp[c * kRadix + tid];  // independent loads, all in flight
     }
-    dbase = block_excl_scan_256(tot, sums, tid);
+    if constexpr (RANK != 0) {
+      if (tid == 0) s_skew = 0;
+    }
+    dbase = block_excl_scan_256(tot, sums, tid);  // contains a barrier
+    if constexpr (RANK != 0) {
+      if (tid < kRadix && tot > n / 32) s_skew = 1;
+    }
     if (tid < kRadix)
       for (uint32_t c = 0; c < (uint32_t)kChains; ++c) nh[c][tid] = 0;
     for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
 #ifdef MGS_RANK_PEERS
     for (int i = tid; i < W * kRadix; i += BLOCK) (&peers[0][0])[i] = 0;
 #endif
+    if constexpr (RANK != 0) {
+      // Not this launch's pass: leave before taking a tile (the whole
+      // workgroup reads the same flag, so it leaves together).
+      __syncthreads();
+      if ((s_skew != 0) != kUni) return;
+    }
     if (tid == taker) s_tile = take_tile(cnt, cm, my_chain, chains_left, chain_step);
     __syncthreads();
   }
@@ -448,7 +475,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
       uint32_t dg[IPT];
 #pragma unroll
       for (int i = 0; i < IPT; ++i) dg[i] = digit_of<KT>(kc[i], shift);
-      rank_items_full<IPT>(dg, ct, pk);
+      rank_items_full<IPT, kUni>(dg, ct, pk);
 #endif
     } else {
 #pragma unroll
@@ -644,7 +671,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
         if (next >= 0) {
 #pragma unroll
           for (int g = 0; g < G; ++g)
-            lds_count(&nh[0][0], chain_of(oo[g], cm) * kRadix + digit_of<KT>(kk[g], 8 * next));
+            lds_count<kUni>(&nh[0][0], chain_of(oo[g], cm) * kRadix + digit_of<KT>(kk[g], 8 * next));
         }
       }
     } else {
@@ -656,7 +683,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
           const uint32_t o = min(gbase[digit_of<KT>(kk, shift)] + i, n - 1);
           dst[o] = kk;
           if constexpr (HV) vdst[o] = vstage[i];
-          if (next >= 0) lds_count(&nh[0][0], chain_of(o, cm) * kRadix + digit_of<KT>(kk, 8 * next));
+          if (next >= 0) lds_count<kUni>(&nh[0][0], chain_of(o, cm) * kRadix + digit_of<KT>(kk, 8 * next));
         }
       }
     }
@@ -1009,6 +1036,19 @@ int xcd_local_allowed() {
   return (e != nullptr && e[0] == '0' && e[1] == 0) ? 0 : 1;
 }
 
+// Smallest sort that launches each pass as a pair of ranking variants
+// (MGS_SKEW_PAIR_MIN keys; the environment variable of the same name, a
+// decimal key count, overrides it).
+size_t skew_pair_min() {
+  const char* e = std::getenv("MGS_SKEW_PAIR_MIN");
+  if (e != nullptr && e[0] != 0) {
+    char* end = nullptr;
+    const unsigned long long v = std::strtoull(e, &end, 10);
+    if (end != nullptr && *end == 0) return (size_t)v;
+  }
+  return (size_t)MGS_SKEW_PAIR_MIN;
+}
+
 struct SortLayout {
   size_t alt_k, alt_v, hist, red, counters, desc, total;
   uint32_t num_tiles;
@@ -1148,6 +1188,21 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
     kernel<<<grid, Cfg::kBlock, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, p, aliased, cm, red, hist, counters,
                                         dsc);
   };
+  // Large sorts launch each pass as a pair of pass_kernel variants, one-
+  // atomic-per-key ranking and uniform-digit ranking; the variant that does
+  // not suit the pass's digit counts exits at once (a few microseconds).
+  const bool rank_pair = BIG && n >= skew_pair_min();
+  auto launch_ranked = [&](auto* dsc, int p) {
+    using DT = std::remove_pointer_t<decltype(dsc)>;
+    if constexpr (BIG) {
+      if (rank_pair) {
+        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, DT, Cfg::kWavesPerEu, 1>, dsc, p);
+        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, DT, Cfg::kWavesPerEu, 2>, dsc, p);
+        return;
+      }
+    }
+    launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, DT, Cfg::kWavesPerEu, 0>, dsc, p);
+  };
   for (int p = 0; p < P; ++p) {
     mark(s);
     if constexpr (PU::kChunkTiles > 1) {
@@ -1157,9 +1212,9 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
         launch_pass(chunk_pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, PU::kChunkTiles, uint32_t>, (uint32_t*)desc, p);
     } else {
       if (L.wide_desc)
-        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint64_t, Cfg::kWavesPerEu>, (uint64_t*)desc, p);
+        launch_ranked(static_cast<uint64_t*>(desc), p);
       else
-        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, uint32_t, Cfg::kWavesPerEu>, (uint32_t*)desc, p);
+        launch_ranked(static_cast<uint32_t*>(desc), p);
     }
   }
   mark(s);

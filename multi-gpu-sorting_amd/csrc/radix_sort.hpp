@@ -191,7 +191,11 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 // with payloads re-checks it, since any other order breaks LSD stability).
 // Tuning switches of the pass kernel (measured on MI355X; see the launch
 // sequence in radix_sort.hip):
-//   MGS_UNI        uniform-digit fast path in ranking and counting
+//   MGS_UNI        uniform-digit fast path in ranking and counting, in
+//                  every pass of every sort
+//   MGS_SKEW_PAIR_MIN  sorts of at least this many keys choose the ranking
+//                  per pass from the digit counts (uniform-digit path when
+//                  one digit holds more than 1/32 of the keys)
 //   MGS_BATCH_WO   write-out of full tiles in batches of 8 items
 //   MGS_SPLIT_LB   1: look-back loads on waves 4..7, descriptor stores on
 //                  0..3; 2: waves 4..7 own the descriptors and finish the
@@ -216,6 +220,9 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 #endif
 #ifndef MGS_BATCH_WO
 #define MGS_BATCH_WO 0
+#endif
+#ifndef MGS_SKEW_PAIR_MIN
+#define MGS_SKEW_PAIR_MIN (1u << 26)
 #endif
 #ifndef MGS_SPLIT_LB
 #define MGS_SPLIT_LB 2
@@ -247,13 +254,13 @@ __device__ __forceinline__ uint32_t rank_wave_atomic(uint32_t d, uint32_t* cnt) 
 // rank_wave_atomic for IPT items of a full wavefront at once: all atomics
 // are issued first, the uniform-digit fix-ups follow, so the LDS round trips
 // overlap.  Returns digit << 16 | rank per item.
-template <int IPT>
+template <int IPT, bool UNI = (MGS_UNI != 0)>
 __device__ __forceinline__ void rank_items_full(const uint32_t (&d)[IPT], uint32_t* cnt, uint32_t (&pk)[IPT]) {
-#if !MGS_UNI
+  if constexpr (!UNI) {
 #pragma unroll
   for (int i = 0; i < IPT; ++i)
     pk[i] = (d[i] << 16) | __hip_atomic_fetch_add(&cnt[d[i]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-#else
+  } else {
   __shared__ uint32_t sink[kWave];
   const uint32_t lane = lane_id();
   uint32_t r[IPT];
@@ -273,16 +280,17 @@ __device__ __forceinline__ void rank_items_full(const uint32_t (&d)[IPT], uint32
     const uint32_t rk = ((uni_mask >> i) & 1u) ? readfirstlane(r[i]) + lane : r[i];
     pk[i] = (d[i] << 16) | rk;
   }
-#endif
+  }
 }
 
 // Counts one item per active lane into LDS counter c[i]; when every active
 // lane counts into the same counter, one atomic adds them all (same
 // branch-free scheme as rank_wave_atomic).
+template <bool UNI = (MGS_UNI != 0)>
 __device__ __forceinline__ void lds_count(uint32_t* c, uint32_t i) {
-#if !MGS_UNI
+  if constexpr (!UNI) {
   atomicAdd(&c[i], 1u);
-#else
+  } else {
   __shared__ uint32_t sink[kWave];
   const uint64_t act = ballot64(true);
   const uint32_t i0 = readfirstlane(i);
@@ -290,7 +298,7 @@ __device__ __forceinline__ void lds_count(uint32_t* c, uint32_t i) {
   const bool lead = !uni || mbcnt64(act) == 0;
   const uint32_t inc = uni ? (lead ? (uint32_t)__popcll(act) : 0u) : 1u;
   __hip_atomic_fetch_add(lead ? &c[i] : &sink[lane_id()], inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-#endif
+  }
 }
 
 // Exclusive scan of one value per digit held by threads 0..255 (4 waves).

@@ -149,6 +149,36 @@ def test_xcd_local_switch_same_result(mg, kt, vdt, monkeypatch):
             assert np.array_equal(from_dev(dv, vals.dtype), ev), mode
 
 
+@pytest.mark.parametrize("dist", ["skewed", "few", "uniform"])
+@pytest.mark.parametrize("kt,vdt", [(O.K_U32, None), (O.K_F32, np.uint32), (O.K_I32, np.uint64), (O.K_U64, None),
+                                    (O.K_F64, np.uint32), (O.K_I64, np.uint64)],
+                         ids=["u32", "f32+v32", "i32+v64", "u64", "f64+v32", "i64+v64"])
+def test_skew_pair_same_result(mg, kt, vdt, dist, monkeypatch):
+    """Large sorts launch every pass as a pair of pass kernels, one-atomic-
+    per-key ranking and uniform-digit ranking, and the one that does not suit
+    the byte's digit counts exits at once.  MGS_SKEW_PAIR_MIN=0 forces the
+    pair on an array below the default size: with bytes whose top digit holds
+    more than 1/32 of the keys (bytes 0 and 2 of "skewed", every byte of
+    "few") and without (uniform), both settings give the oracle's result on
+    one temp buffer, payloads stable."""
+    import torch
+    rng = np.random.default_rng(131 + kt)
+    n = (1 << 21) + 54_321  # large tiles for every record width
+    keys = random_keys(kt, n, rng, dist)
+    vals = None if vdt is None else random_vals(vdt, n, rng)
+    ek, ev = O.sort(kt, keys, vals)
+    vt = mg.VAL_NONE if vdt is None else (mg.VAL_U32 if vdt == np.uint32 else mg.VAL_U64)
+    temp = mg.temp_buffer(mg.lib.mgs_sort_temp_bytes(kt, vt, n), "cuda")
+    for pair_min in ("0", str(1 << 26), "0"):
+        monkeypatch.setenv("MGS_SKEW_PAIR_MIN", pair_min)
+        dk, dv = to_dev(keys), to_dev(vals)
+        mg.sort(KEY_NAME[kt], dk, dv, temp=temp)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(dk, keys.dtype), ek), pair_min
+        if vdt is not None:
+            assert np.array_equal(from_dev(dv, vals.dtype), ev), pair_min
+
+
 @pytest.mark.parametrize("kt,vdt", [(O.K_U64, None), (O.K_I64, np.uint64), (O.K_U32, np.uint32), (O.K_F64, np.uint32),
                                     (O.K_I32, np.uint64)],
                          ids=["u64", "i64+v64", "u32+v32", "f64+v32", "i32+v64"])
