@@ -32,6 +32,20 @@ constexpr uint32_t kLookbackSpinLimit = MGS_LB_SPIN_LIMIT;
 #define MGS_RANK(d, peers, cnt) rank_wave_atomic((d), (cnt))
 #endif
 
+// The segmented kernels rank and count with the uniform-digit path (one
+// atomic for a wavefront whose lanes share a digit) unless MGS_SEG_UNI=0:
+// measured on 4096 arrays of u64 keys + u32 payload, it costs 1.7% on
+// uniform keys and saves 35% on keys with 1000 distinct values.
+#ifndef MGS_SEG_UNI
+#define MGS_SEG_UNI 1
+#endif
+constexpr bool kSegUni = MGS_SEG_UNI != 0;
+#ifdef MGS_RANK_PEERS
+#define MGS_SEG_RANK(d, peers, cnt) rank_wave((d), (peers), (cnt))
+#else
+#define MGS_SEG_RANK(d, peers, cnt) rank_wave_atomic<kSegUni>((d), (cnt))
+#endif
+
 constexpr int kWindow = MGS_LB_WINDOW;
 constexpr int kPassWindow = MGS_PASS_LB_WINDOW;
 
@@ -196,11 +210,11 @@ __device__ __forceinline__ void onesweep_tile(const typename Codec<KT>::Bits* __
   (void)pt;
   if (full) {
 #pragma unroll
-    for (int k = 0; k < IPT; ++k) rk[k] = MGS_RANK(dg[k], pt, ct);
+    for (int k = 0; k < IPT; ++k) rk[k] = MGS_SEG_RANK(dg[k], pt, ct);
   } else {
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
-      if (wb + k * kWave < n) rk[k] = MGS_RANK(dg[k], pt, ct);
+      if (wb + k * kWave < n) rk[k] = MGS_SEG_RANK(dg[k], pt, ct);
     }
   }
   __syncthreads();

@@ -233,10 +233,11 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 // 64 conflicting ones.  Branch-free, so that the atomics of consecutive items
 // issue back to back: in the uniform case the first lane adds the lane count
 // and the others add 0 to a private slot of `sink`.
+template <bool UNI = (MGS_UNI != 0)>
 __device__ __forceinline__ uint32_t rank_wave_atomic(uint32_t d, uint32_t* cnt) {
-#if !MGS_UNI
+  if constexpr (!UNI) {
   return __hip_atomic_fetch_add(&cnt[d], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-#else
+  } else {
   __shared__ uint32_t sink[kWave];
   const uint64_t act = ballot64(true);
   const uint32_t d0 = readfirstlane(d);
@@ -248,7 +249,7 @@ __device__ __forceinline__ uint32_t rank_wave_atomic(uint32_t d, uint32_t* cnt) 
                                             __HIP_MEMORY_SCOPE_WORKGROUP);
   const uint32_t r0 = readfirstlane(r);
   return uni ? r0 + below : r;
-#endif
+  }
 }
 
 // rank_wave_atomic for IPT items of a full wavefront at once: all atomics
