@@ -36,6 +36,10 @@ def random_keys(kt: int, n: int, rng: np.random.Generator, dist: str = "uniform"
         half = rng.random(n) < 0.5
         k[half] = (k[half] & ~np.uint64(0xFF0000)) | np.uint64(0x550000)
         k = k.astype(bd)
+    elif dist == "byte1":  # byte 0 constant, byte 1 takes 3 values, the higher bytes are uniform
+        k = random_keys(kt, n, rng).astype(np.uint64)
+        k = (k & ~np.uint64(0xFFFF)) | np.uint64(0x34) | (rng.integers(0, 3, size=n).astype(np.uint64) << np.uint64(8))
+        k = k.astype(bd)
     elif dist == "lowbits":  # only the low byte varies
         k = (rng.integers(0, 256, size=n).astype(np.uint64) | np.uint64(0x1200)).astype(bd)
     else:

@@ -149,7 +149,7 @@ def test_xcd_local_switch_same_result(mg, kt, vdt, monkeypatch):
             assert np.array_equal(from_dev(dv, vals.dtype), ev), mode
 
 
-@pytest.mark.parametrize("dist", ["skewed", "few", "uniform"])
+@pytest.mark.parametrize("dist", ["skewed", "few", "uniform", "byte1"])
 @pytest.mark.parametrize("kt,vdt", [(O.K_U32, None), (O.K_F32, np.uint32), (O.K_I32, np.uint64), (O.K_U64, None),
                                     (O.K_F64, np.uint32), (O.K_I64, np.uint64)],
                          ids=["u32", "f32+v32", "i32+v64", "u64", "f64+v32", "i64+v64"])
@@ -159,8 +159,10 @@ def test_skew_pair_same_result(mg, kt, vdt, dist, monkeypatch):
     the byte's digit counts exits at once.  MGS_SKEW_PAIR_MIN=0 forces the
     pair on an array below the default size: with bytes whose top digit holds
     more than 1/32 of the keys (bytes 0 and 2 of "skewed", every byte of
-    "few") and without (uniform), both settings give the oracle's result on
-    one temp buffer, payloads stable."""
+    "few") and without (uniform), and with a constant byte 0 below a skewed
+    byte 1 (counted from the input by the first kernel of the pair only),
+    both settings give the oracle's result on one temp buffer, payloads
+    stable."""
     import torch
     rng = np.random.default_rng(131 + kt)
     n = (1 << 21) + 54_321  # large tiles for every record width
