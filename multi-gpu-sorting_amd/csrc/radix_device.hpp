@@ -32,10 +32,11 @@ constexpr uint32_t kLookbackSpinLimit = MGS_LB_SPIN_LIMIT;
 #define MGS_RANK(d, peers, cnt) rank_wave_atomic((d), (cnt))
 #endif
 
-// The segmented kernels rank and count with the uniform-digit path (one
-// atomic for a wavefront whose lanes share a digit) unless MGS_SEG_UNI=0:
-// measured on 4096 arrays of u64 keys + u32 payload, it costs 1.7% on
-// uniform keys and saves 35% on keys with 1000 distinct values.
+// The segmented kernels rank and count with the uniform-digit path (the
+// lanes of a wavefront that share the first lane's digit take one atomic)
+// unless MGS_SEG_UNI=0: measured on 4096 arrays of u64 keys + u32 payload,
+// it costs 1.7% on uniform keys and saves 35% on keys with 1000 distinct
+// values.
 #ifndef MGS_SEG_UNI
 #define MGS_SEG_UNI 1
 #endif

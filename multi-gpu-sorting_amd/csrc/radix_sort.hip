@@ -309,8 +309,8 @@ __device__ __forceinline__ uint32_t chain_digit_base(const uint32_t* __restrict_
 //   0  one atomic per key (the only variant launched for smaller sorts);
 //   1  the same, as the first of a pair of launches for one pass: exits at
 //      once when the pass is skewed;
-//   2  uniform-digit ranking (a wavefront whose lanes share one digit takes
-//      one LDS atomic instead of 64 conflicting ones), the second of the
+//   2  uniform-digit ranking (the lanes of a wavefront that share the first
+//      lane's digit take one LDS atomic instead of one each), the second of the
 //      pair: exits at once unless the pass is skewed.
 // A pass is skewed when one digit of its byte holds more than 1/32 of the
 // keys (heavy duplicates).  The choice is made by the pair of launches, not

@@ -228,32 +228,36 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 #define MGS_SPLIT_LB 2
 #endif
 
-// A wavefront whose active lanes all carry one digit (runs of equal bytes in
-// sorted or nearly sorted input) takes one atomic for all of them instead of
-// 64 conflicting ones.  Branch-free, so that the atomics of consecutive items
-// issue back to back: in the uniform case the first lane adds the lane count
-// and the others add 0 to a private slot of `sink`.
+// With UNI (the uniform-digit path), the lanes that carry the first active
+// lane's digit (the frequent digit of heavy-duplicate keys, or the only one
+// in runs of equal bytes) take one atomic for the whole group instead of one
+// each: the first active lane adds the group's size, the other group lanes
+// add 0 to a private slot of `sink`, and each group lane's rank is the first
+// lane's result plus the number of group lanes below it.  The remaining lanes
+// take one atomic each, on other counters.  Ranks stay in lane order within
+// every digit.  Branch-free, so that the atomics of consecutive items issue
+// back to back.
 template <bool UNI = (MGS_UNI != 0)>
 __device__ __forceinline__ uint32_t rank_wave_atomic(uint32_t d, uint32_t* cnt) {
   if constexpr (!UNI) {
   return __hip_atomic_fetch_add(&cnt[d], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else {
   __shared__ uint32_t sink[kWave];
-  const uint64_t act = ballot64(true);
   const uint32_t d0 = readfirstlane(d);
-  const bool uni = ballot64(d == d0) == act;
-  const uint32_t below = mbcnt64(act);
-  const bool lead = !uni || below == 0;
-  const uint32_t inc = uni ? (below == 0 ? (uint32_t)__popcll(act) : 0u) : 1u;
+  const bool in = d == d0;  // the group of the first active lane's digit
+  const uint64_t grp = ballot64(in);
+  const uint32_t below = mbcnt64(grp);
+  const bool lead = !in || below == 0;
+  const uint32_t inc = in ? (below == 0 ? (uint32_t)__popcll(grp) : 0u) : 1u;
   const uint32_t r = __hip_atomic_fetch_add(lead ? &cnt[d] : &sink[lane_id()], inc, __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_WORKGROUP);
   const uint32_t r0 = readfirstlane(r);
-  return uni ? r0 + below : r;
+  return in ? r0 + below : r;
   }
 }
 
 // rank_wave_atomic for IPT items of a full wavefront at once: all atomics
-// are issued first, the uniform-digit fix-ups follow, so the LDS round trips
+// are issued first, the digit-group fix-ups follow, so the LDS round trips
 // overlap.  Returns digit << 16 | rank per item.
 template <int IPT, bool UNI = (MGS_UNI != 0)>
 __device__ __forceinline__ void rank_items_full(const uint32_t (&d)[IPT], uint32_t* cnt, uint32_t (&pk)[IPT]) {
@@ -265,39 +269,37 @@ __device__ __forceinline__ void rank_items_full(const uint32_t (&d)[IPT], uint32
   __shared__ uint32_t sink[kWave];
   const uint32_t lane = lane_id();
   uint32_t r[IPT];
-  uint32_t uni_mask = 0;  // wave-uniform
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
-    const uint32_t d0 = readfirstlane(d[i]);
-    const bool uni = ballot64(d[i] == d0) == ~0ull;
-    uni_mask |= uni ? (1u << i) : 0u;
-    const bool lead = !uni || lane == 0;
-    const uint32_t inc = uni ? (lane == 0 ? (uint32_t)kWave : 0u) : 1u;
+    const bool in = d[i] == readfirstlane(d[i]);  // lane 0's digit group
+    const uint64_t grp = ballot64(in);
+    const bool lead = !in || lane == 0;
+    const uint32_t inc = in ? (lane == 0 ? (uint32_t)__popcll(grp) : 0u) : 1u;
     r[i] = __hip_atomic_fetch_add(lead ? &cnt[d[i]] : &sink[lane], inc, __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
-    const uint32_t rk = ((uni_mask >> i) & 1u) ? readfirstlane(r[i]) + lane : r[i];
+    const bool in = d[i] == readfirstlane(d[i]);
+    const uint32_t rk = in ? readfirstlane(r[i]) + mbcnt64(ballot64(in)) : r[i];
     pk[i] = (d[i] << 16) | rk;
   }
   }
 }
 
-// Counts one item per active lane into LDS counter c[i]; when every active
-// lane counts into the same counter, one atomic adds them all (same
-// branch-free scheme as rank_wave_atomic).
+// Counts one item per active lane into LDS counter c[i]; with UNI, the lanes
+// that count into the first active lane's counter add with one atomic for
+// the group (same branch-free scheme as rank_wave_atomic).
 template <bool UNI = (MGS_UNI != 0)>
 __device__ __forceinline__ void lds_count(uint32_t* c, uint32_t i) {
   if constexpr (!UNI) {
   atomicAdd(&c[i], 1u);
   } else {
   __shared__ uint32_t sink[kWave];
-  const uint64_t act = ballot64(true);
-  const uint32_t i0 = readfirstlane(i);
-  const bool uni = ballot64(i == i0) == act;
-  const bool lead = !uni || mbcnt64(act) == 0;
-  const uint32_t inc = uni ? (lead ? (uint32_t)__popcll(act) : 0u) : 1u;
+  const bool in = i == readfirstlane(i);  // the group of the first active lane's counter
+  const uint64_t grp = ballot64(in);
+  const bool lead = !in || mbcnt64(grp) == 0;
+  const uint32_t inc = in ? (lead ? (uint32_t)__popcll(grp) : 0u) : 1u;
   __hip_atomic_fetch_add(lead ? &c[i] : &sink[lane_id()], inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }
