@@ -41,6 +41,19 @@ constexpr uint32_t kLookbackSpinLimit = MGS_LB_SPIN_LIMIT;
 #define MGS_SEG_UNI 1
 #endif
 constexpr bool kSegUni = MGS_SEG_UNI != 0;
+// The segment histograms (every byte of every key counted once) count with
+// MGS_SEG_HIST_UNI = 0: one atomic per key; 1: lds_count<true> (digit
+// groups); 2: lds_count_wave (one atomic for a wavefront that shares one
+// counter).
+#ifndef MGS_SEG_HIST_UNI
+#define MGS_SEG_HIST_UNI 2
+#endif
+__device__ __forceinline__ void seg_hist_count(uint32_t* c, uint32_t i) {
+  if constexpr (MGS_SEG_HIST_UNI == 2)
+    lds_count_wave(c, i);
+  else
+    lds_count<MGS_SEG_HIST_UNI != 0>(c, i);
+}
 #ifdef MGS_RANK_PEERS
 #define MGS_SEG_RANK(d, peers, cnt) rank_wave((d), (peers), (cnt))
 #else

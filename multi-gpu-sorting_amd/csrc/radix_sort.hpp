@@ -304,6 +304,20 @@ __device__ __forceinline__ void lds_count(uint32_t* c, uint32_t i) {
   }
 }
 
+// Counts one item per active lane into LDS counter c[i]: one atomic when
+// every active lane counts into the same counter (a wave-uniform branch),
+// else one atomic per lane.  Cheaper per count than lds_count<true> when
+// most wavefronts are mixed, as good when whole wavefronts share a counter.
+__device__ __forceinline__ void lds_count_wave(uint32_t* c, uint32_t i) {
+  const uint32_t i0 = readfirstlane(i);
+  const uint64_t act = ballot64(true);
+  if (ballot64(i == i0) == act) {
+    if (mbcnt64(act) == 0) atomicAdd(&c[i0], (uint32_t)__popcll(act));
+  } else {
+    atomicAdd(&c[i], 1u);
+  }
+}
+
 // Exclusive scan of one value per digit held by threads 0..255 (4 waves).
 // `sums` is 4 words of LDS.  Must be called by every thread of the block
 // (contains barriers); threads >= 256 pass 0 and get garbage.

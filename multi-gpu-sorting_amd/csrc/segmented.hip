@@ -276,7 +276,7 @@ __device__ void medium_sort_block(const typename Codec<KT>::Bits* k_in, typename
   for (uint32_t i = tid; i < n; i += BLOCK) {
     const Bits k = k_in[i];
 #pragma unroll
-    for (int p = 0; p < P; ++p) lds_count<kSegUni>(&hist[0][0], p * kRadix + digit_of<KT>(k, 8 * p));
+    for (int p = 0; p < P; ++p) seg_hist_count(&hist[0][0], p * kRadix + digit_of<KT>(k, 8 * p));
   }
   __syncthreads();
 
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(BLOCK) void seg_hist_kernel(const typename Codec<KT
   for (uint32_t i = t0 + tid; i < t1; i += BLOCK) {
     const Bits k = k_in[b + i];
 #pragma unroll
-    for (int p = 0; p < P; ++p) lds_count<kSegUni>(&h[0][0], p * kRadix + digit_of<KT>(k, 8 * p));
+    for (int p = 0; p < P; ++p) seg_hist_count(&h[0][0], p * kRadix + digit_of<KT>(k, 8 * p));
   }
   __syncthreads();
   uint32_t* dsth = seg_hist + (size_t)li * P * kRadix;
