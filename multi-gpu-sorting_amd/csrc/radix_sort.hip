@@ -312,8 +312,8 @@ __device__ __forceinline__ uint32_t chain_digit_base(const uint32_t* __restrict_
 //   2  uniform-digit ranking (the lanes of a wavefront that share the first
 //      lane's digit take one LDS atomic instead of one each), the second of the
 //      pair: exits at once unless the pass is skewed.
-// A pass is skewed when one digit of its byte holds more than 1/32 of the
-// keys (heavy duplicates).  The choice is made by the pair of launches, not
+// A pass is skewed when one digit of its byte holds more than
+// n >> MGS_SKEW_SHIFT of the keys (heavy duplicates).  The choice is made by the pair of launches, not
 // by a branch inside one kernel, because a kernel holding both ranking paths
 // needs more registers than one workgroup of 1024 threads per CU can give
 // and spills.
@@ -420,7 +420,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     }
     dbase = block_excl_scan_256(tot, sums, tid);  // contains a barrier
     if constexpr (RANK != 0) {
-      if (tid < kRadix && tot > n / 32) s_skew = 1;
+      if (tid < kRadix && tot > (n >> MGS_SKEW_SHIFT)) s_skew = 1;
     }
     if (tid < kRadix)
       for (uint32_t c = 0; c < (uint32_t)kChains; ++c) nh[c][tid] = 0;

@@ -195,7 +195,8 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 //                  every pass of every sort
 //   MGS_SKEW_PAIR_MIN  sorts of at least this many keys choose the ranking
 //                  per pass from the digit counts (uniform-digit path when
-//                  one digit holds more than 1/32 of the keys)
+//                  one digit holds more than n >> MGS_SKEW_SHIFT keys,
+//                  1/64 of them by default)
 //   MGS_BATCH_WO   write-out of full tiles in batches of 8 items
 //   MGS_SPLIT_LB   1: look-back loads on waves 4..7, descriptor stores on
 //                  0..3; 2: waves 4..7 own the descriptors and finish the
@@ -223,6 +224,9 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 #endif
 #ifndef MGS_SKEW_PAIR_MIN
 #define MGS_SKEW_PAIR_MIN (1u << 26)
+#endif
+#ifndef MGS_SKEW_SHIFT
+#define MGS_SKEW_SHIFT 6
 #endif
 #ifndef MGS_SPLIT_LB
 #define MGS_SPLIT_LB 2

@@ -30,7 +30,7 @@ def random_keys(kt: int, n: int, rng: np.random.Generator, dist: str = "uniform"
         k = k[np.argsort(O.encode(kt, k), kind="stable")]
     elif dist == "reverse":
         k = random_keys(kt, n, rng, "sorted")[::-1].copy()
-    elif dist == "skewed":  # bytes 0 and 2 have a digit with > 1/32 of the keys, the others are uniform
+    elif dist == "skewed":  # bytes 0 and 2 have a digit with > 1/64 of the keys, the others are uniform
         k = random_keys(kt, n, rng).astype(np.uint64)
         k = (k & ~np.uint64(0xFF)) | rng.integers(0, 4, size=n).astype(np.uint64)
         half = rng.random(n) < 0.5

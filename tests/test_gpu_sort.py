@@ -158,7 +158,7 @@ def test_skew_pair_same_result(mg, kt, vdt, dist, monkeypatch):
     per-key ranking and uniform-digit ranking, and the one that does not suit
     the byte's digit counts exits at once.  MGS_SKEW_PAIR_MIN=0 forces the
     pair on an array below the default size: with bytes whose top digit holds
-    more than 1/32 of the keys (bytes 0 and 2 of "skewed", every byte of
+    more than 1/64 of the keys (bytes 0 and 2 of "skewed", every byte of
     "few") and without (uniform), and with a constant byte 0 below a skewed
     byte 1 (counted from the input by the first kernel of the pair only),
     both settings give the oracle's result on one temp buffer, payloads
