@@ -60,18 +60,31 @@ enum {
   MGS_ERR_TEMP_TOO_SMALL = -2, /* temp_bytes below the *_temp_bytes() answer  */
   MGS_ERR_HIP = -3,            /* a HIP call failed; see mgs_last_hip_error() */
   MGS_ERR_INVALID = -4,        /* bad argument (NULL pointer, size, overlap)  */
-  MGS_ERR_TOO_LARGE = -5       /* n beyond what the kernels index (2^32 - 1)  */
+  MGS_ERR_TOO_LARGE = -5,      /* n above MGS_MAX_KEYS                        */
+  MGS_ERR_DEVICE = -6,         /* a kernel gave up a bounded wait or found a  */
+                               /* look-back chain not fully served: the       */
+                               /* output is not valid (mgs_temp_status)       */
+  MGS_ERR_SKIPPED = -7         /* segments left unsorted: malformed bounds,  */
+                               /* beyond max_total_keys or max_segment_keys   */
+                               /* (mgs_temp_status)                           */
 };
 
-/* Tuning of the radix kernels; a zeroed struct (or NULL) selects defaults
- * from the built-in table chosen by measurement on MI355X. */
+/* Largest key count of one call: the kernels index keys with 32 bits and
+ * the last tile (at most 65536 keys) must end below 2^32. */
+#define MGS_MAX_KEYS ((size_t)0xFFFF0000u)
+
+/* Tuning of the segmented sort; a zeroed struct (or NULL) selects the
+ * defaults chosen by measurement on MI355X. */
 typedef struct mgs_tuning {
-  int32_t tile_keys;     /* keys per onesweep tile (0 = default)            */
-  int32_t small_max;     /* segments up to this length sort inside one      */
-                         /* workgroup's LDS (0 = default)                   */
-  int32_t medium_max;    /* segments up to this length sort with a single   */
-                         /* workgroup streaming HBM (0 = default)           */
-  int32_t reserved;
+  int32_t large_variant;    /* tile geometry of the large segments' passes: */
+                            /* 0 = default, 1.. = mgs_segmented_variant()   */
+  int32_t small_max;        /* segments up to this length sort inside one   */
+                            /* workgroup's LDS (0 = default)                */
+  int32_t medium_max;       /* segments up to this length sort with a single*/
+                            /* workgroup streaming HBM (0 = default)        */
+  int32_t max_segment_keys; /* longest segment the call accepts (0 = any up */
+                            /* to max_total_keys); below 2^30 the look-back */
+                            /* descriptors take 4 bytes instead of 8        */
 } mgs_tuning;
 
 /* --------------------------------------------------------------------------
@@ -82,7 +95,12 @@ typedef struct mgs_tuning {
  *
  * Sorts n keys (and, if vt != MGS_VAL_NONE, their payloads) stably by enc().
  * keys_in may equal keys_out (in place, thrust::sort semantics) and vals_in
- * may equal vals_out; otherwise the inputs are only read.  n <= 2^32 - 1.
+ * may equal vals_out; otherwise the inputs are only read.  n <= MGS_MAX_KEYS.
+ *
+ * Scratch: mgs_sort_temp_bytes(kt, vt, n) <= n * (sizeof key + sizeof payload)
+ * + 128,000,000 for every n <= MGS_MAX_KEYS (the reference's device buffer
+ * plan, src/device_buffers.cuh:30-44), and it does not decrease with n, so a
+ * buffer sized for n also serves every smaller sort.
  * ------------------------------------------------------------------------ */
 size_t mgs_sort_temp_bytes(mgs_key kt, mgs_val vt, size_t n);
 
@@ -102,6 +120,20 @@ int mgs_sort_timed(mgs_key kt, mgs_val vt,
                    const void* vals_in, void* vals_out,
                    size_t n, void* temp, size_t temp_bytes, void* stream,
                    void* const* events, int num_events, int* num_marks);
+
+/* Which algorithm the last mgs_sort / mgs_sort_timed call with these
+ * arguments and this temp buffer used (no reference counterpart): waits for
+ * `stream` and returns MGS_PATH_SMALL (one workgroup in LDS, no scratch),
+ * MGS_PATH_LSD (onesweep LSD passes) or MGS_PATH_THREE_PASS (the MSD pass,
+ * the bucket pass and the LDS sort of sub-buckets, tried for 4-byte keys
+ * alone from 2^26 keys; it hands over to the LSD passes when the key
+ * distribution does not suit it), or a negative status. */
+enum { MGS_PATH_SMALL = 1, MGS_PATH_LSD = 2, MGS_PATH_THREE_PASS = 3 };
+int mgs_sort_path(mgs_key kt, mgs_val vt, size_t n, const void* temp, size_t temp_bytes, void* stream);
+
+/* Comma-separated names of the stages of this thread's last mgs_sort_timed
+ * call, one per mark but the last (no reference counterpart). */
+const char* mgs_sort_timed_stages(void);
 
 /* --------------------------------------------------------------------------
  * Merge-path merge.  Replaces thrust::merge at reference
@@ -166,6 +198,12 @@ int mgs_fill_max(mgs_key kt, void* keys, size_t n, void* stream);
 size_t mgs_segmented_sort_temp_bytes(mgs_key kt, mgs_val vt,
                                      size_t max_total_keys, size_t nseg);
 
+/* The same for calls that pass `tuning` (its max_segment_keys shrinks the
+ * look-back descriptors); NULL gives mgs_segmented_sort_temp_bytes. */
+size_t mgs_segmented_sort_temp_bytes_tuned(mgs_key kt, mgs_val vt,
+                                           size_t max_total_keys, size_t nseg,
+                                           const mgs_tuning* tuning);
+
 int mgs_segmented_sort(mgs_key kt, mgs_val vt,
                        const void* keys_in, void* keys_out,
                        const void* vals_in, void* vals_out,
@@ -189,6 +227,17 @@ int mgs_segmented_sort_ranges(mgs_key kt, mgs_val vt,
                               size_t nseg,
                               size_t max_total_keys, const mgs_tuning* tuning,
                               void* temp, size_t temp_bytes, void* stream);
+
+/* Device-side status of the last mgs_sort / mgs_sort_timed /
+ * mgs_segmented_sort / mgs_segmented_sort_ranges call that used `temp` (no
+ * reference counterpart: the reference ignores device errors,
+ * src/cuda_error.cuh:5-10).  Those calls' kernels record what they detect
+ * while running in the first word of temp: a look-back wait that reached its
+ * bound, a look-back chain whose tiles were not all taken, or segments left
+ * unsorted.  Waits for `stream`, then returns MGS_OK, MGS_ERR_DEVICE,
+ * MGS_ERR_SKIPPED or MGS_ERR_HIP.  Calls that needed no scratch
+ * (temp_bytes == 0) always succeed on the device: pass temp = NULL. */
+int mgs_temp_status(const void* temp, size_t temp_bytes, void* stream);
 
 /* Status text; never NULL. */
 const char* mgs_status_str(int status);

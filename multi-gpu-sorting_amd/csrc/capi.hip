@@ -1,5 +1,6 @@
 // extern "C" entry points of libmgsort (declared in include/mgsort.h).
 #include "internal.hpp"
+#include "radix_sort.hpp"
 
 namespace mgs {
 
@@ -22,7 +23,8 @@ using mgs::val_bytes;
 
 namespace {
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
-constexpr size_t kMaxKeys = 0xFFFFFFFFull;  // kernels index with 32 bits
+constexpr size_t kMaxKeys = MGS_MAX_KEYS;  // kernels index with 32 bits, tiles of <= 65536 keys
+static_assert(kMaxKeys + 65536 <= (size_t(1) << 32), "the last tile must end below 2^32");
 }  // namespace
 
 extern "C" {
@@ -52,6 +54,12 @@ int mgs_sort_timed(mgs_key kt, mgs_val vt, const void* keys_in, void* keys_out, 
   if (vt != MGS_VAL_NONE && (vals_in == nullptr || vals_out == nullptr)) return MGS_ERR_INVALID;
   return mgs::sort_dispatch(kt, vt, keys_in, keys_out, vals_in, vals_out, n, temp, temp_bytes, as_stream(stream),
                             reinterpret_cast<hipEvent_t*>(const_cast<void**>(events)), num_events, num_marks);
+}
+
+const char* mgs_sort_timed_stages(void) { return mgs::last_stage_names(); }
+
+int mgs_sort_path(mgs_key kt, mgs_val vt, size_t n, const void* temp, size_t temp_bytes, void* stream) {
+  return mgs::sort_path(kt, vt, n, temp, temp_bytes, as_stream(stream));
 }
 
 size_t mgs_merge_temp_bytes(mgs_key kt, mgs_val vt, size_t na, size_t nb) {
@@ -88,7 +96,12 @@ int mgs_fill_max(mgs_key kt, void* keys, size_t n, void* stream) {
 }
 
 size_t mgs_segmented_sort_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total_keys, size_t nseg) {
-  return mgs::segmented_temp_bytes(kt, vt, max_total_keys, nseg);
+  return mgs::segmented_temp_bytes(kt, vt, max_total_keys, nseg, nullptr);
+}
+
+size_t mgs_segmented_sort_temp_bytes_tuned(mgs_key kt, mgs_val vt, size_t max_total_keys, size_t nseg,
+                                           const mgs_tuning* tuning) {
+  return mgs::segmented_temp_bytes(kt, vt, max_total_keys, nseg, tuning);
 }
 
 int mgs_segmented_sort(mgs_key kt, mgs_val vt, const void* keys_in, void* keys_out, const void* vals_in,
@@ -115,6 +128,17 @@ int mgs_segmented_sort_ranges(mgs_key kt, mgs_val vt, void* keys, void* vals, co
                                  temp_bytes, as_stream(stream));
 }
 
+int mgs_temp_status(const void* temp, size_t temp_bytes, void* stream) {
+  if (temp == nullptr || temp_bytes < sizeof(uint32_t)) return MGS_OK;
+  uint32_t word = 0;
+  hipError_t e = hipMemcpyAsync(&word, temp, sizeof(word), hipMemcpyDeviceToHost, as_stream(stream));
+  if (e == hipSuccess) e = hipStreamSynchronize(as_stream(stream));
+  if (e != hipSuccess) return mgs::record_hip(e);
+  if (word & (mgs::kStatusLookback | mgs::kStatusChainUnserved)) return MGS_ERR_DEVICE;
+  if (word & mgs::kStatusSkipped) return MGS_ERR_SKIPPED;
+  return MGS_OK;
+}
+
 const char* mgs_status_str(int status) {
   switch (status) {
     case MGS_OK: return "ok";
@@ -123,6 +147,8 @@ const char* mgs_status_str(int status) {
     case MGS_ERR_HIP: return "HIP runtime error";
     case MGS_ERR_INVALID: return "invalid argument";
     case MGS_ERR_TOO_LARGE: return "too many keys";
+    case MGS_ERR_DEVICE: return "a kernel detected an invalid look-back state; output not valid";
+    case MGS_ERR_SKIPPED: return "segments left unsorted (malformed or out-of-bound segments)";
   }
   return "unknown status";
 }

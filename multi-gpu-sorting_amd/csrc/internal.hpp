@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <string>
 
 #include "../../include/mgsort.h"
 
@@ -15,6 +16,22 @@ namespace mgs {
 // MGS_OK when e == hipSuccess.
 int record_hip(hipError_t e);
 int last_hip_error();
+
+// Stage marks of a launch sequence (mgs_sort_timed): records events[i], when
+// given, before stage i, and collects the stage names.
+struct Marks {
+  hipEvent_t* ev;
+  int cap, used;
+  std::string* names;
+  void operator()(hipStream_t s, const char* name = nullptr) {
+    if (ev != nullptr && used < cap) (void)hipEventRecord(ev[used], s);
+    if (names != nullptr && name != nullptr) {
+      if (!names->empty()) names->push_back(',');
+      names->append(name);
+    }
+    ++used;
+  }
+};
 
 // Checks the launch(es) just enqueued.
 inline int check_launch() { return record_hip(hipGetLastError()); }
@@ -40,6 +57,12 @@ int sort_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const voi
                   void* temp, size_t temp_bytes, hipStream_t s, hipEvent_t* events = nullptr, int num_events = 0,
                   int* num_marks = nullptr);
 
+// mgs_sort_path.
+int sort_path(mgs_key kt, mgs_val vt, size_t n, const void* temp, size_t temp_bytes, hipStream_t s);
+
+// Comma-separated stage names of this thread's last mgs_sort_timed call.
+const char* last_stage_names();
+
 size_t merge_temp_bytes(mgs_key kt, mgs_val vt, size_t na, size_t nb);
 int merge_dispatch(mgs_key kt, mgs_val vt, const void* ak, const void* av, size_t na, const void* bk, const void* bv,
                    size_t nb, void* ok, void* ov, void* temp, size_t temp_bytes, hipStream_t s);
@@ -48,7 +71,7 @@ int pivot_dispatch(mgs_key kt, const void* const* left, const void* const* right
 int swap_dispatch(mgs_key kt, mgs_val vt, void* ak, void* av, void* bk, void* bv, size_t n, hipStream_t s);
 int fill_max_dispatch(mgs_key kt, void* keys, size_t n, hipStream_t s);
 
-size_t segmented_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total, size_t nseg);
+size_t segmented_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total, size_t nseg, const mgs_tuning* tuning);
 // ranges == false: d_bounds holds nseg+1 offsets; true: nseg (begin, end)
 // pairs, with payloads at d_val_begin[s] (NULL: at the keys' begin).
 int segmented_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout,

@@ -114,34 +114,38 @@ __device__ __forceinline__ bool lookback_step(const D* desc, int64_t& t, uint32_
 
 template <typename D, int NW>
 __device__ __forceinline__ uint32_t lookback_from(const D* desc, int64_t t, uint32_t d, int parity,
-                                                  D (&v)[NW], uint32_t* stats = nullptr, uint32_t prefix = 0) {
-  uint32_t spins = 0, rounds = 1;
+                                                  D (&v)[NW], uint32_t prefix, uint32_t spin_limit, bool& gave_up) {
+  uint32_t spins = 0;
+  gave_up = false;
   for (;;) {
     bool done;
     const int consumed = lookback_consume(t, parity, v, prefix, done);
     if (done || t - consumed < 0) break;
     if (consumed == 0) {
-      // Bounded wait: a predecessor tile always publishes within microseconds;
-      // the bound only keeps a corrupted descriptor array from hanging the GPU
-      // (the result is then wrong, every store stays in bounds).
-      if (++spins > kLookbackSpinLimit) break;
+      // Bounded wait: a predecessor tile always publishes within microseconds.
+      // The bound keeps a corrupted descriptor array (or a chain no
+      // workgroup serves) from hanging the GPU; the caller records it in the
+      // status word, since the prefix is then incomplete (every store stays
+      // in bounds).
+      if (++spins > spin_limit) {
+        gave_up = true;
+        break;
+      }
       __builtin_amdgcn_s_sleep(1);
     }
     t -= consumed;
     lookback_load(desc, t, d, v);
-    ++rounds;
   }
-  if (stats) *stats = rounds | (spins << 16);
   return prefix;
 }
 
 // Look-back of tile `tile` (> 0) of a chain for digit d.
 template <typename D>
 __device__ __forceinline__ uint32_t lookback(const D* desc, uint32_t tile, uint32_t d, int parity,
-                                             uint32_t* stats = nullptr) {
+                                             uint32_t spin_limit, bool& gave_up) {
   D v[kWindow];
   lookback_load(desc, (int64_t)tile - 1, d, v);
-  return lookback_from(desc, (int64_t)tile - 1, d, parity, v, stats);
+  return lookback_from(desc, (int64_t)tile - 1, d, parity, v, 0u, spin_limit, gave_up);
 }
 
 // ---------------------------------------------------------------------------
@@ -158,13 +162,14 @@ __device__ __forceinline__ uint32_t lookback(const D* desc, uint32_t tile, uint3
 // tile        tile index inside the chain
 // desc        the chain's descriptors (tile-major, 256 per tile)
 // digit_base  256 output offsets of the chain's first key of each digit
+// status      the call's status word (kStatusLookback when a wait gives up)
 // ---------------------------------------------------------------------------
 template <int KT, typename V, int BLOCK, int IPT, typename D>
 __device__ __forceinline__ void onesweep_tile(const typename Codec<KT>::Bits* __restrict__ src,
                                               typename Codec<KT>::Bits* __restrict__ dst, const V* __restrict__ vsrc,
                                               V* __restrict__ vdst, uint32_t n, uint32_t n_dst, uint32_t tile,
                                               D* __restrict__ desc, const uint32_t* __restrict__ digit_base, int shift,
-                                              int parity) {
+                                              int parity, uint32_t* __restrict__ status) {
   using Bits = typename Codec<KT>::Bits;
   constexpr int W = BLOCK / kWave;
   constexpr int TILE = BLOCK * IPT;
@@ -252,7 +257,9 @@ __device__ __forceinline__ void onesweep_tile(const typename Codec<KT>::Bits* __
     }
     uint32_t prefix = 0;
     if (tile > 0) {
-      prefix = lookback(desc, tile, (uint32_t)tid, parity);
+      bool gave_up;
+      prefix = lookback(desc, tile, (uint32_t)tid, parity, kLookbackSpinLimit, gave_up);
+      if (gave_up) atomicOr(status, kStatusLookback);
       st_agent(my_desc, Desc<D>::make(kStInclusive, (D)(prefix + total), parity));
     }
     gbase[tid] = digit_base[tid] + prefix - excl;

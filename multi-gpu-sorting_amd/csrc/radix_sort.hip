@@ -7,6 +7,7 @@
 #include <utility>
 
 #include "internal.hpp"
+#include "msd_sort.hpp"
 #include "radix_device.hpp"
 
 namespace mgs {
@@ -41,12 +42,20 @@ namespace mgs {
 
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;
 
+// gate: NULL, or the decision word of the three-pass path launched before
+// this sort (msd_sort.hip); the LSD kernels then run only when it is nonzero
+// (the path turned itself off).
+__device__ __forceinline__ bool gated_off(const uint32_t* gate) { return gate != nullptr && *gate == 0; }
+
 __global__ void init_kernel(uint32_t* __restrict__ hist, uint32_t words, uint32_t* __restrict__ counters,
-                            uint32_t counter_words, uint64_t* __restrict__ red) {
+                            uint32_t counter_words, uint64_t* __restrict__ red, uint32_t* __restrict__ status,
+                            const uint32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t i = g; i < words; i += gridDim.x * blockDim.x) hist[i] = 0;
   for (uint32_t i = g; i < counter_words; i += gridDim.x * blockDim.x) counters[i] = 0;
   if (g == 0) {
+    *status = 0;
     red[0] = 0;     // OR identity
     red[1] = ~0ull;  // AND identity
     red[2] = 0;      // XCDs seen by the upfront kernel
@@ -82,8 +91,9 @@ template <int KT, int BLOCK, bool VEC>
 __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>::Bits* __restrict__ keys, uint32_t n,
                                                         uint32_t* __restrict__ hist, uint64_t* __restrict__ red,
                                                         uint32_t* __restrict__ desc, size_t desc_words, ChainMap cm,
-                                                        int record_xcd) {
+                                                        int record_xcd, const uint32_t* __restrict__ gate) {
   using Bits = typename Codec<KT>::Bits;
+  if (gated_off(gate)) return;
   __shared__ uint32_t h[kRadix];
   __shared__ Bits s_or[BLOCK / kWave], s_and[BLOCK / kWave];
   const int tid = threadIdx.x;
@@ -270,19 +280,6 @@ __device__ __forceinline__ void publish(D* p, D v, bool xcd_local) {
     st_agent(p, v);
 }
 
-#ifdef MGS_PHASE_PROBE
-__device__ uint64_t* g_phase_probe;
-#define PHASE(slot)                                                                                 \
-  do {                                                                                              \
-    if (g_phase_probe && tid == 0 && (tile & 63) == 0)                                              \
-      g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 16 + (slot)] = __builtin_amdgcn_s_memrealtime(); \
-  } while (0)
-#else
-#define PHASE(slot) \
-  do {              \
-  } while (0)
-#endif
-
 // ---------------------------------------------------------------------------
 // One onesweep pass over byte `pass`, persistent workgroups, chained tiles.
 // hist is [P][kChains][256]; counters is [P][kChains].
@@ -305,33 +302,51 @@ __device__ __forceinline__ uint32_t chain_digit_base(const uint32_t* __restrict_
   return b;
 }
 
-// RANK selects how digits are ranked and counted in LDS:
-//   0  one atomic per key (the only variant launched for smaller sorts);
-//   1  the same, as the first of a pair of launches for one pass: exits at
-//      once when the pass is skewed;
+// RANK selects how each wavefront ranks its keys' digits in LDS:
+//   0  one atomic per key (ds_add_rtn_u32 on the wavefront's counter of the
+//      digit; the only variant launched for smaller sorts);
+//   1  the same, first of a pair of launches for one pass: exits at once
+//      when the pass is skewed;
 //   2  uniform-digit ranking (the lanes of a wavefront that share the first
-//      lane's digit take one LDS atomic instead of one each), the second of the
-//      pair: exits at once unless the pass is skewed.
+//      lane's digit take one LDS atomic instead of one each), second of the
+//      pair: exits at once unless the pass is skewed;
+//   3  peer-mask ranking (rank_wave), chosen at run time with
+//      MGS_RANK_PEERS=1 for 4-byte keys.
+// Variants 0-2 are stable because the LDS serves the conflicting lanes of one
+// ds_add_rtn_u32 in ascending lane order: measured on gfx950 (and re-checked
+// by every multi-pass parity test with payloads), not a documented guarantee.
+// Variant 3 relies only on the in-order execution of one wavefront's DS
+// instructions, at the cost of an OR, a read and a clear per key.
 // A pass is skewed when one digit of its byte holds more than
-// n >> MGS_SKEW_SHIFT of the keys (heavy duplicates).  The choice is made by the pair of launches, not
-// by a branch inside one kernel, because a kernel holding both ranking paths
-// needs more registers than one workgroup of 1024 threads per CU can give
-// and spills.
+// n >> MGS_SKEW_SHIFT of the keys (heavy duplicates).  The choice is made by
+// the pair of launches, not by a branch inside one kernel, because a kernel
+// holding both ranking paths needs more registers than one workgroup of 1024
+// threads per CU can give and spills.
+//
+// Per tile, waves 4..7 (threads 256..511, one per digit) own the look-back:
+// they publish the tile's digit counts, take the next tile and issue the
+// first look-back window while waves 0..3 scan the counts into tile-local
+// offsets; they take one more look-back round before the scatter and finish
+// after it, so the look-back latency overlaps the scan and the scatter.
+// Every wavefront loads its part of the next tile as soon as its keys of the
+// current tile are in LDS.
 template <int KT, typename V, int BLOCK, int IPT, typename D, int WAVES, int RANK = 0>
 __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
     const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, ChainMap cm,
     const uint64_t* __restrict__ red, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters,
-    D* __restrict__ desc) {
+    D* __restrict__ desc, uint32_t* __restrict__ status, uint32_t spin_limit, const uint32_t* __restrict__ gate) {
   using Bits = typename Codec<KT>::Bits;
   constexpr int P = (int)sizeof(Bits);
   constexpr int W = BLOCK / kWave;
   constexpr int TILE = BLOCK * IPT;
   constexpr bool HV = kHasVal<V>;
   using VS = typename std::conditional<HV, V, uint32_t>::type;
-  static_assert(W >= 4, "need >= 256 threads for the per-digit phase");
+  static_assert(W >= 8, "the look-back wavefronts 4..7 need >= 512 threads");
 
   constexpr bool kUni = RANK == 2;
+  constexpr bool kPeers = RANK == 3;
+  if (gated_off(gate)) return;
   const uint32_t vary = varying_bytes(red, P);
   if (!(vary & (1u << pass))) {
     if constexpr (kUni) return;  // the RANK 1 launch of the pair does this
@@ -361,39 +376,22 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
 
   __shared__ Bits keys[TILE];
   __shared__ VS vstage[HV ? TILE : 1];
-#ifdef MGS_RANK_PEERS
-  __shared__ uint64_t peers[W][kRadix];
-#endif
+  __shared__ uint64_t peers[kPeers ? W : 1][kRadix];
   __shared__ uint32_t off[W][kRadix];
   __shared__ uint32_t nh[kChains][kRadix];
   __shared__ uint32_t gbase[kRadix];
   __shared__ uint32_t s_prefix[kRadix];
   __shared__ uint32_t sums[4];
   __shared__ uint32_t s_tile;
-  // With eight or more wavefronts the look-back loads run on waves 4..7 and
-  // the descriptor stores on waves 0..3, so no wavefront waits for its own
-  // store to complete before it can use a look-back load.
-  constexpr bool kSplitLookback = MGS_SPLIT_LB && W >= 8;
-  // MGS_SPLIT_LB == 2: waves 4..7 own the descriptors.  They publish the
-  // tile aggregate and issue the first look-back window before the digit
-  // scan, and finish the look-back (inclusive descriptor, digit bases) after
-  // the tile has been scattered into LDS, so its latency overlaps the scan
-  // and the scatter.
-  constexpr bool kDeferLB = kSplitLookback && MGS_SPLIT_LB == 2;
-  // MGS_EARLY_PF (with the deferred look-back): the next tile is taken while
-  // the digits are scanned, and each wavefront loads its part of it as soon
-  // as its keys of the current tile are in LDS.
-  constexpr bool kEarlyPF = kDeferLB && MGS_EARLY_PF;
-  // MGS_LB_STEP (with the deferred look-back): one look-back round is taken
-  // before the scatter, so the next round trip overlaps the scatter.
-  constexpr bool kLbStep = kDeferLB && MGS_LB_STEP;
+  __shared__ uint32_t s_skew;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int w = tid / kWave;
-  // Tiles are taken by one thread of a wavefront that issues no descriptor
-  // stores (its atomic then waits for nothing else); its acquisition state:
-  const int taker = kSplitLookback ? kRadix : 0;
+  const bool lb_wave = tid >= kRadix && tid < 2 * kRadix;
+  // Tiles are taken by the first look-back thread (its wavefront issues no
+  // descriptor stores before the atomic); its acquisition state:
+  const int taker = kRadix;
   const bool xcd_local = MGS_XCD_LOCAL && cm.chains % kXcds == 0 && ld_agent(&red[2]) == (1ull << kXcds) - 1;
   uint32_t my_chain = blockIdx.x % cm.chains, chains_left = cm.chains, chain_step = 1;
   if (xcd_local) {
@@ -407,7 +405,6 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
   // of each digit; the base of a chain adds the keys of earlier chains with
   // the same digit, recomputed when the workgroup moves to another chain.
   uint32_t dbase = 0, cur_chain = 0xFFFFFFFFu, cur_base = 0;
-  __shared__ uint32_t s_skew;
   {
     uint32_t tot = 0;
     if (tid < kRadix) {
@@ -415,20 +412,20 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
       for (uint32_t c = 0; c < (uint32_t)kChains; ++c)
         if (c < cm.chains) tot += hp[c * kRadix + tid];  // independent loads, all in flight
     }
-    if constexpr (RANK != 0) {
+    if constexpr (RANK == 1 || RANK == 2) {
       if (tid == 0) s_skew = 0;
     }
     dbase = block_excl_scan_256(tot, sums, tid);  // contains a barrier
-    if constexpr (RANK != 0) {
+    if constexpr (RANK == 1 || RANK == 2) {
       if (tid < kRadix && tot > (n >> MGS_SKEW_SHIFT)) s_skew = 1;
     }
     if (tid < kRadix)
       for (uint32_t c = 0; c < (uint32_t)kChains; ++c) nh[c][tid] = 0;
     for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
-#ifdef MGS_RANK_PEERS
-    for (int i = tid; i < W * kRadix; i += BLOCK) (&peers[0][0])[i] = 0;
-#endif
-    if constexpr (RANK != 0) {
+    if constexpr (kPeers) {
+      for (int i = tid; i < W * kRadix; i += BLOCK) (&peers[0][0])[i] = 0;
+    }
+    if constexpr (RANK == 1 || RANK == 2) {
       // Not this launch's pass: leave before taking a tile (the whole
       // workgroup reads the same flag, so it leaves together).
       __syncthreads();
@@ -446,7 +443,6 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
   }
 
   while (tile != kNoTile) {
-    PHASE(1);
     const uint32_t chain = tile / cm.tpc;
     const uint32_t first = chain * cm.tpc;
     const uint32_t lt = tile - first;  // tile index inside its chain
@@ -458,30 +454,31 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     // that digit (< IPT * 64)
     uint32_t pk[IPT];
     uint32_t* ct = off[w];
-#ifdef MGS_RANK_PEERS
-    uint64_t* pt = peers[w];
-#else
-    uint64_t* pt = nullptr;
-#endif
-    (void)pt;
     if (full) {
-#ifdef MGS_RANK_PEERS
+      if constexpr (kPeers) {
 #pragma unroll
-      for (int i = 0; i < IPT; ++i) {
-        const uint32_t d = digit_of<KT>(kc[i], shift);
-        pk[i] = (d << 16) | MGS_RANK(d, pt, ct);
+        for (int i = 0; i < IPT; ++i) {
+          const uint32_t d = digit_of<KT>(kc[i], shift);
+          pk[i] = (d << 16) | rank_wave(d, peers[w], ct);
+        }
+      } else {
+        uint32_t dg[IPT];
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) dg[i] = digit_of<KT>(kc[i], shift);
+        rank_items_full<IPT, kUni>(dg, ct, pk);
       }
-#else
-      uint32_t dg[IPT];
-#pragma unroll
-      for (int i = 0; i < IPT; ++i) dg[i] = digit_of<KT>(kc[i], shift);
-      rank_items_full<IPT, kUni>(dg, ct, pk);
-#endif
     } else {
 #pragma unroll
       for (int i = 0; i < IPT; ++i) {
         const uint32_t d = digit_of<KT>(kc[i], shift);
-        pk[i] = (wb + i * kWave < n) ? ((d << 16) | MGS_RANK(d, pt, ct)) : 0u;
+        if (wb + i * kWave < n) {
+          if constexpr (kPeers)
+            pk[i] = (d << 16) | rank_wave(d, peers[w], ct);
+          else
+            pk[i] = (d << 16) | rank_wave_atomic<false>(d, ct);
+        } else {
+          pk[i] = 0u;
+        }
       }
     }
     // The keys are in registers now; tell the compiler so, so that the
@@ -489,35 +486,22 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
 #pragma unroll
     for (int i = 0; i < IPT; ++i) asm volatile("" : "+v"(kc[i]));
     __syncthreads();
-    PHASE(2);
 
     uint32_t total = 0;
-    D* my_desc = desc + (size_t)tile * kRadix + tid;
     const D* chain_desc = desc + (size_t)first * kRadix;
     D win[kPassWindow];
     uint32_t taken = kNoTile;
-    if (kDeferLB && tid >= kRadix && tid < 2 * kRadix) {
+    if (lb_wave) {
       const uint32_t d = (uint32_t)tid - kRadix;
-      if (kEarlyPF && tid == taker) taken = take_tile(cnt, cm, my_chain, chains_left, chain_step);
+      if (tid == taker) taken = take_tile(cnt, cm, my_chain, chains_left, chain_step);
 #pragma unroll
       for (int i = 0; i < W; ++i) total += off[i][d];
-      publish(desc + (size_t)tile * kRadix + d, Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)total, parity), xcd_local);
+      publish(desc + (size_t)tile * kRadix + d, Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)total, parity),
+              xcd_local);
       if (lt > 0) lookback_load(chain_desc, (int64_t)lt - 1, d, win);
     } else if (tid < kRadix) {
 #pragma unroll
       for (int i = 0; i < W; ++i) total += off[i][tid];
-      if (!kSplitLookback && lt > 0) lookback_load(chain_desc, (int64_t)lt - 1, (uint32_t)tid, win);
-      if (!kDeferLB) publish(my_desc, Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)total, parity), xcd_local);
-    } else if (kSplitLookback && tid < 2 * kRadix) {
-      const uint32_t d = (uint32_t)tid - kRadix;
-#ifdef MGS_PHASE_PROBE
-      uint32_t st = 0;
-      s_prefix[d] = lt > 0 ? lookback(chain_desc, lt, d, parity, &st) : 0u;
-      if (d == 0 && g_phase_probe && (tile & 63) == 0)
-        g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 16 + 7] = st;
-#else
-      s_prefix[d] = lt > 0 ? lookback(chain_desc, lt, d, parity) : 0u;
-#endif
     }
     const uint32_t excl = block_excl_scan_256(tid < kRadix ? total : 0u, sums, tid);
     if (tid < kRadix) {
@@ -528,45 +512,18 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
         off[i][tid] = run;
         run += c;
       }
-      if (!kDeferLB) {
-        uint32_t prefix = 0;
-        if (lt > 0) {
-          prefix = kSplitLookback ? s_prefix[tid] : lookback_from(chain_desc, (int64_t)lt - 1, (uint32_t)tid, parity, win);
-          publish(my_desc, Desc<D>::make(kStInclusive, (D)(prefix + total), parity), xcd_local);
-        }
-        if (chain != cur_chain) {
-          cur_chain = chain;
-          cur_base = chain_digit_base(hp, dbase, chain, tid);
-        }
-        gbase[tid] = cur_base + prefix - excl;
-      } else {
-        if (chain != cur_chain) {
-          cur_chain = chain;
-          cur_base = chain_digit_base(hp, dbase, chain, tid);
-        }
-        s_prefix[tid] = cur_base - excl;  // the digit base without the look-back prefix
+      if (chain != cur_chain) {
+        cur_chain = chain;
+        cur_base = chain_digit_base(hp, dbase, chain, tid);
       }
-      if (tid == 0) PHASE(4);
+      s_prefix[tid] = cur_base - excl;  // the digit base without the look-back prefix
     }
-    if (kEarlyPF && tid == taker) s_tile = taken;
+    if (tid == taker) s_tile = taken;
     __syncthreads();
-    PHASE(3);
-    // Without the early prefetch the next tile is taken only now, after
-    // this tile's inclusive prefixes are published (with it, the taker took
-    // it during the digit scan, so that its keys can be loaded right after
-    // the scatter).
-    if (!kEarlyPF && tid == taker) {
-      s_tile = take_tile(cnt, cm, my_chain, chains_left, chain_step);
-#ifdef MGS_PHASE_PROBE
-      if (g_phase_probe && (tile & 63) == 0)
-        g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 16] = __builtin_amdgcn_s_memrealtime();
-#endif
-    }
     int64_t lb_t = (int64_t)lt - 1;
     uint32_t lb_prefix = 0;
     bool lb_done = false;
-    if (kLbStep && tid >= kRadix && tid < 2 * kRadix && lt > 0)
-      lb_done = lookback_step(chain_desc, lb_t, (uint32_t)tid - kRadix, parity, win, lb_prefix);
+    if (lb_wave && lt > 0) lb_done = lookback_step(chain_desc, lb_t, (uint32_t)tid - kRadix, parity, win, lb_prefix);
 
     if (full) {
       // all offset reads first, then all writes (branch-free; the clamp only
@@ -589,107 +546,49 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
         }
       }
     }
-    PHASE(8);
+    // This wavefront's rank counters and keys are free: clear the counters
+    // and load its part of the next tile.
     Bits kn[IPT];
     VS vn[IPT];
-    uint32_t nxt = kNoTile;
-    // The look-back wavefronts (4..7) load their part of the next tile only
-    // after the look-back when MGS_EARLY_PF is 2: loads complete in issue
-    // order, so a look-back round issued after the tile loads would wait for
-    // them.
-    const bool lb_wave = kDeferLB && tid >= kRadix && tid < 2 * kRadix;
-    const bool pf_late = MGS_EARLY_PF == 2 && lb_wave && lt > 0;
-    if (kEarlyPF) {
-      // this wavefront's rank counters and keys are free: clear the counters
-      // and load its part of the next tile
-      nxt = s_tile;
-      if (nxt != kNoTile && !pf_late) {
-        const uint32_t tb = nxt * (uint32_t)TILE;
-        load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kn, vn);
-      }
-      for (int i = lane; i < kRadix; i += kWave) off[w][i] = 0;
-    }
-    if (kDeferLB && tid >= kRadix && tid < 2 * kRadix && lt > 0) {
-      // this wavefront's part of the tile is in LDS: finish the look-back
-      const uint32_t d = (uint32_t)tid - kRadix;
-#ifdef MGS_PHASE_PROBE
-      uint32_t st = 0;
-      const uint32_t prefix = lb_done ? lb_prefix : lookback_from(chain_desc, lb_t, d, parity, win, &st, lb_prefix);
-      if (d == 0 && g_phase_probe && (tile & 63) == 0)
-        g_phase_probe[((size_t)pass * 65536 + (tile >> 6)) * 16 + 7] = st + (kLbStep ? 1 : 0);
-#else
-      const uint32_t prefix = lb_done ? lb_prefix : lookback_from(chain_desc, lb_t, d, parity, win, nullptr, lb_prefix);
-#endif
-      publish(desc + (size_t)tile * kRadix + d, Desc<D>::make(kStInclusive, (D)(prefix + total), parity), xcd_local);
-      gbase[d] = s_prefix[d] + prefix;
-    } else if (kDeferLB && tid >= kRadix && tid < 2 * kRadix) {
-      gbase[tid - kRadix] = s_prefix[tid - kRadix];
-    }
-    if (kEarlyPF && pf_late && nxt != kNoTile) {
+    const uint32_t nxt = s_tile;
+    if (nxt != kNoTile) {
       const uint32_t tb = nxt * (uint32_t)TILE;
       load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kn, vn);
     }
-    __syncthreads();
-    PHASE(9);
-    if (!kEarlyPF) {
-      // prefetch the next tile while this one is written out; the rank
-      // counters are free again and are cleared for it
-      nxt = s_tile;
-      if (nxt != kNoTile) {
-        const uint32_t tb = nxt * (uint32_t)TILE;
-        load_tile<KT, V, IPT>(src, vsrc, n, tb + (uint32_t)(w * IPT * kWave + lane), n - tb >= (uint32_t)TILE, kn, vn);
+    for (int i = lane; i < kRadix; i += kWave) off[w][i] = 0;
+    if (lb_wave) {
+      // this wavefront's part of the tile is in LDS: finish the look-back
+      const uint32_t d = (uint32_t)tid - kRadix;
+      if (lt > 0) {
+        bool gave_up = false;
+        const uint32_t prefix =
+            lb_done ? lb_prefix : lookback_from(chain_desc, lb_t, d, parity, win, lb_prefix, spin_limit, gave_up);
+        if (gave_up) atomicOr(status, kStatusLookback);
+        publish(desc + (size_t)tile * kRadix + d, Desc<D>::make(kStInclusive, (D)(prefix + total), parity), xcd_local);
+        gbase[d] = s_prefix[d] + prefix;
+      } else {
+        gbase[d] = s_prefix[d];
       }
-      for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
     }
-    PHASE(10);
+    __syncthreads();
 
     // Write-out: consecutive threads store consecutive positions of each
     // digit run.  o < n for every key of a well-formed pass; the clamp only
     // keeps a corrupted prefix in bounds and leaves the stores unconditional.
-    // Full tiles go in batches of G items, LDS reads of a batch issued
-    // together, then the stores, then the counting for the next byte.
-    if (MGS_BATCH_WO && full) {
-      constexpr int G = IPT < 8 ? IPT : 8;
+    // On the way out each key is counted in the next varying byte's
+    // histogram of the chain its output position falls in.
 #pragma unroll
-      for (int j0 = 0; j0 < IPT; j0 += G) {
-        Bits kk[G];
-        VS vv[G];
-        uint32_t oo[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          kk[g] = keys[(j0 + g) * BLOCK + tid];
-          if constexpr (HV) vv[g] = vstage[(j0 + g) * BLOCK + tid];
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-          oo[g] = min(gbase[digit_of<KT>(kk[g], shift)] + (uint32_t)((j0 + g) * BLOCK + tid), n - 1);
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          dst[oo[g]] = kk[g];
-          if constexpr (HV) vdst[oo[g]] = vv[g];
-        }
-        if (next >= 0) {
-#pragma unroll
-          for (int g = 0; g < G; ++g)
-            lds_count<kUni>(&nh[0][0], chain_of(oo[g], cm) * kRadix + digit_of<KT>(kk[g], 8 * next));
-        }
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        const uint32_t i = (uint32_t)(j * BLOCK + tid);
-        if (full || i < tile_cnt) {
-          const Bits kk = keys[i];
-          const uint32_t o = min(gbase[digit_of<KT>(kk, shift)] + i, n - 1);
-          dst[o] = kk;
-          if constexpr (HV) vdst[o] = vstage[i];
-          if (next >= 0) lds_count<kUni>(&nh[0][0], chain_of(o, cm) * kRadix + digit_of<KT>(kk, 8 * next));
-        }
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t i = (uint32_t)(j * BLOCK + tid);
+      if (full || i < tile_cnt) {
+        const Bits kk = keys[i];
+        const uint32_t o = min(gbase[digit_of<KT>(kk, shift)] + i, n - 1);
+        dst[o] = kk;
+        if constexpr (HV) vdst[o] = vstage[i];
+        if (next >= 0) lds_count<kUni>(&nh[0][0], chain_of(o, cm) * kRadix + digit_of<KT>(kk, 8 * next));
       }
     }
-    PHASE(5);
     __syncthreads();
-    PHASE(6);
     tile = nxt;
 #pragma unroll
     for (int i = 0; i < IPT; ++i) {
@@ -704,293 +603,28 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
   }
 }
 
-// ---------------------------------------------------------------------------
-// One onesweep pass in chunks of CT tiles, persistent workgroups.
-//
-// A workgroup takes a chunk (CT consecutive tiles of one chain) and first
-// counts the digits of the whole chunk (phase H, last tile first so that the
-// first tile's keys stay in registers), publishes the chunk's counts and
-// issues the look-back loads.  It then ranks, scatters and writes the tiles
-// one after another with running per-digit offsets (phase S); the look-back
-// result is needed only by the first write-out, so its latency is spent
-// ranking and scattering the first tile, and there is one look-back per
-// chunk instead of one per tile.  The later tiles of the chunk are read a
-// second time a few microseconds after phase H (from the caches in the usual
-// case).
-// ---------------------------------------------------------------------------
-#ifndef MGS_CHUNK_TILES
-#define MGS_CHUNK_TILES 1
-#endif
-
-template <int KT, int IPT>
-__device__ __forceinline__ void load_keys(const typename Codec<KT>::Bits* __restrict__ src, uint32_t n, uint32_t wb,
-                                          bool full, typename Codec<KT>::Bits (&k)[IPT]) {
-  using Bits = typename Codec<KT>::Bits;
-  if (full) {
-#pragma unroll
-    for (int i = 0; i < IPT; ++i) k[i] = src[wb + i * kWave];
-  } else {
-#pragma unroll
-    for (int i = 0; i < IPT; ++i) {
-      const uint32_t idx = wb + i * kWave;
-      k[i] = idx < n ? src[idx] : Bits(0);
-    }
-  }
-}
-
-template <int KT, typename V, int BLOCK, int IPT, int CT, typename D>
-__global__ __launch_bounds__(BLOCK, BLOCK / 128) void chunk_pass_kernel(
-    const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
-    const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, ChainMap cm,
-    const uint64_t* __restrict__ red, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters,
-    D* __restrict__ desc) {
-  using Bits = typename Codec<KT>::Bits;
-  constexpr int P = (int)sizeof(Bits);
-  constexpr int W = BLOCK / kWave;
-  constexpr int TILE = BLOCK * IPT;
-  constexpr uint32_t CHUNK = (uint32_t)TILE * CT;
-  constexpr bool HV = kHasVal<V>;
-  using VS = typename std::conditional<HV, V, uint32_t>::type;
-  static_assert(W >= 8, "the look-back runs on waves 4..7");
-
-  const uint32_t vary = varying_bytes(red, P);
-  if (!(vary & (1u << pass))) {
-    const uint32_t below = vary & ((2u << pass) - 1u);
-    if (below == 0 && pass + 1 < P && (vary & (1u << (pass + 1))))
-      histogram_only<KT, BLOCK>(k_in, n, pass + 1, hist + (size_t)(pass + 1) * kChains * kRadix, cm);
-    return;
-  }
-  const int k = __builtin_popcount(vary & ((1u << pass) - 1u));
-  const int na = __builtin_popcount(vary);
-  const uint32_t above = vary & ~((2u << pass) - 1u);
-  const int next = above ? __builtin_ctz(above) : -1;
-  const int sb = src_of(k, na, aliased), db = dst_of(k, na, aliased);
-  const Bits* __restrict__ src = pick_src(sb, k_in, (const Bits*)k_alt, (const Bits*)k_out);
-  Bits* __restrict__ dst = pick_dst(db, k_alt, k_out);
-  const V* __restrict__ vsrc = nullptr;
-  V* __restrict__ vdst = nullptr;
-  if constexpr (HV) {
-    vsrc = pick_src(sb, v_in, (const V*)v_alt, (const V*)v_out);
-    vdst = pick_dst(db, v_alt, v_out);
-  }
-  const int shift = 8 * pass, parity = k & 1;
-  uint32_t* __restrict__ cnt = counters + (size_t)pass * kChains;
-  const uint32_t* __restrict__ hp = hist + (size_t)pass * kChains * kRadix;
-
-  __shared__ Bits keys[TILE];
-  __shared__ VS vstage[HV ? TILE : 1];
-  __shared__ uint32_t off[W][kRadix];   // rank counters of the current tile
-  __shared__ uint32_t hcnt[W][kRadix];  // digit counts of the current chunk
-  __shared__ uint32_t nh[kChains][kRadix];
-  __shared__ uint32_t gbase[kRadix];
-  __shared__ uint32_t s_prefix[kRadix];
-  __shared__ uint32_t sums[4];
-  __shared__ uint32_t s_chunk;
-
-  const int tid = threadIdx.x;
-  const int lane = tid & (kWave - 1);
-  const int w = tid / kWave;
-  const int taker = kRadix;  // wave 4: issues no descriptor stores
-  uint32_t my_chain = blockIdx.x % cm.chains, chains_left = cm.chains;
-
-  uint32_t dbase = 0, cur_chain = 0xFFFFFFFFu, cur_base = 0;
-  {
-    uint32_t tot = 0;
-    if (tid < kRadix) {
-#pragma unroll
-      for (uint32_t c = 0; c < (uint32_t)kChains; ++c)
-        if (c < cm.chains) tot += hp[c * kRadix + tid];
-    }
-    dbase = block_excl_scan_256(tot, sums, tid);
-    if (tid < kRadix)
-      for (uint32_t c = 0; c < (uint32_t)kChains; ++c) nh[c][tid] = 0;
-    for (int i = tid; i < W * kRadix; i += BLOCK) {
-      (&off[0][0])[i] = 0;
-      (&hcnt[0][0])[i] = 0;
-    }
-    if (tid == taker) s_chunk = take_tile(cnt, cm, my_chain, chains_left);
-    __syncthreads();
-  }
-  uint32_t chunk = s_chunk;
-
-  while (chunk != kNoTile) {
-    const uint32_t chain = chunk / cm.tpc;
-    const uint32_t first = chain * cm.tpc;
-    const uint32_t lc = chunk - first;  // chunk index inside its chain
-    const uint32_t cb = chunk * CHUNK;
-    const uint32_t ccnt = min(CHUNK, n - cb);
-    const uint32_t ntiles = (ccnt + TILE - 1) / TILE;
-    const uint32_t lw = (uint32_t)(w * IPT * kWave + lane);
-
-    // ---- phase H: digit counts of the chunk, last tile first; the loads of
-    // the next tile down are issued before the current one is counted (a
-    // runtime loop, so the compiler keeps two tiles in registers, not CT) ----
-    Bits kc[IPT], kp[IPT];
-    VS vc[IPT];
-    {
-      int tt = (int)ntiles - 1;
-      uint32_t tb = cb + (uint32_t)tt * TILE;
-      bool full = n - tb >= (uint32_t)TILE;
-      if (tt == 0)
-        load_tile<KT, V, IPT>(src, vsrc, n, tb + lw, full, kc, vc);
-      else
-        load_keys<KT, IPT>(src, n, tb + lw, full, kc);
-#pragma unroll 1
-      for (; tt >= 0; --tt) {
-        if (tt >= 1) {  // tiles below the chunk's last are full
-          if (tt == 1)
-            load_tile<KT, V, IPT>(src, vsrc, n, tb - TILE + lw, true, kp, vc);
-          else
-            load_keys<KT, IPT>(src, n, tb - TILE + lw, true, kp);
-        }
-#pragma unroll
-        for (int i = 0; i < IPT; ++i)
-          if (full || tb + lw + i * kWave < n) atomicAdd(&hcnt[w][digit_of<KT>(kc[i], shift)], 1u);
-        if (tt >= 1) {
-#pragma unroll
-          for (int i = 0; i < IPT; ++i) kc[i] = kp[i];
-          tb -= TILE;
-          full = true;
-        }
-      }
-    }
-    __syncthreads();
-
-    // publish the chunk's counts; waves 4..7 issue the look-back loads
-    uint32_t ctot = 0;
-    D* my_desc = desc + (size_t)chunk * kRadix + tid;
-    const D* chain_desc = desc + (size_t)first * kRadix;
-    D win[kWindow];
-    if (tid < kRadix) {
-#pragma unroll
-      for (int i = 0; i < W; ++i) {
-        ctot += hcnt[i][tid];
-        hcnt[i][tid] = 0;  // ready for the next chunk (read only here)
-      }
-      st_agent(my_desc, Desc<D>::make(lc == 0 ? kStInclusive : kStAggregate, (D)ctot, parity));
-      if (chain != cur_chain) {
-        cur_chain = chain;
-        cur_base = chain_digit_base(hp, dbase, chain, tid);
-      }
-    } else if (tid < 2 * kRadix) {
-      if (lc > 0) lookback_load(chain_desc, (int64_t)lc - 1, (uint32_t)(tid - kRadix), win);
-    }
-
-    // ---- phase S: tiles in order, running offsets per digit ----
-    uint32_t run = 0;  // per digit (tid < 256): next free output slot
-    for (uint32_t tt = 0; tt < ntiles; ++tt) {
-      const uint32_t tb = cb + tt * TILE;
-      const uint32_t tile_cnt = min((uint32_t)TILE, n - tb);
-      const bool full = tile_cnt == (uint32_t)TILE;
-      const uint32_t wb = tb + lw;
-      uint32_t pk[IPT];
-      uint32_t* ct = off[w];
-      if (full) {
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-          const uint32_t d = digit_of<KT>(kc[i], shift);
-          pk[i] = (d << 16) | __hip_atomic_fetch_add(&ct[d], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-          const uint32_t d = digit_of<KT>(kc[i], shift);
-          pk[i] = (wb + i * kWave < n) ? ((d << 16) | rank_wave_atomic(d, ct)) : 0u;
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < IPT; ++i) asm volatile("" : "+v"(kc[i]));
-      const bool last = tt + 1 == ntiles;
-      if (last && tid == taker) s_chunk = take_tile(cnt, cm, my_chain, chains_left);
-      __syncthreads();
-
-      uint32_t total = 0;
-      if (tid < kRadix) {
-#pragma unroll
-        for (int i = 0; i < W; ++i) total += off[i][tid];
-      }
-      const uint32_t excl = block_excl_scan_256(total, sums, tid);
-      if (tid < kRadix) {
-        uint32_t r = excl;
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-          const uint32_t c = off[i][tid];
-          off[i][tid] = r;
-          r += c;
-        }
-      }
-      if (tt == 0) {
-        // scatter first, then finish the look-back and publish the prefix
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-          const uint32_t pos = off[w][pk[i] >> 16] + (pk[i] & 0xFFFFu);
-          if ((full || wb + i * kWave < n) && pos < (uint32_t)TILE) {
-            keys[pos] = kc[i];
-            if constexpr (HV) vstage[pos] = vc[i];
-          }
-        }
-        if (tid >= kRadix && tid < 2 * kRadix)
-          s_prefix[tid - kRadix] = lc > 0 ? lookback_from(chain_desc, (int64_t)lc - 1, (uint32_t)(tid - kRadix), parity, win) : 0u;
-        __syncthreads();
-        if (tid < kRadix) {
-          const uint32_t prefix = s_prefix[tid];
-          if (lc > 0) st_agent(my_desc, Desc<D>::make(kStInclusive, (D)(prefix + ctot), parity));
-          run = cur_base + prefix;
-          gbase[tid] = run - excl;
-          run += total;
-        }
-      } else {
-        if (tid < kRadix) {
-          gbase[tid] = run - excl;
-          run += total;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-          const uint32_t pos = off[w][pk[i] >> 16] + (pk[i] & 0xFFFFu);
-          if ((full || wb + i * kWave < n) && pos < (uint32_t)TILE) {
-            keys[pos] = kc[i];
-            if constexpr (HV) vstage[pos] = vc[i];
-          }
-        }
-      }
-      __syncthreads();
-
-      // prefetch the chunk's next tile while this one is written out
-      if (!last) {
-        const uint32_t nb = tb + TILE;
-        load_tile<KT, V, IPT>(src, vsrc, n, nb + lw, n - nb >= (uint32_t)TILE, kc, vc);
-      }
-      for (int i = tid; i < W * kRadix; i += BLOCK) (&off[0][0])[i] = 0;
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        const uint32_t i = (uint32_t)(j * BLOCK + tid);
-        if (full || i < tile_cnt) {
-          const Bits kk = keys[i];
-          const uint32_t o = min(gbase[digit_of<KT>(kk, shift)] + i, n - 1);
-          dst[o] = kk;
-          if constexpr (HV) vdst[o] = vstage[i];
-          if (next >= 0) lds_count(&nh[0][0], chain_of(o, cm) * kRadix + digit_of<KT>(kk, 8 * next));
-        }
-      }
-      __syncthreads();
-    }
-    chunk = s_chunk;
-  }
-  if (next >= 0 && tid < kRadix) {
-    uint32_t* hn = hist + (size_t)next * kChains * kRadix;
-    for (uint32_t c = 0; c < cm.chains; ++c)
-      if (nh[c][tid]) atomicAdd(&hn[c * kRadix + tid], nh[c][tid]);
-  }
-}
-
 // Copies the result into the output buffers when the passes left it in the
-// scratch buffer (or did not run).
+// scratch buffer (or did not run), and checks that every varying byte's pass
+// took every tile of every chain (a chain no workgroup served -- say, an XCD
+// without resident workgroups in XCD-local mode -- would leave its keys
+// unwritten): take_tile() counts every attempt, so a chain's counter ends at
+// or above its tile count iff all of its tiles were taken.
 template <typename Bits, typename V>
 __global__ void fixup_kernel(const Bits* k_in, const Bits* k_alt, Bits* k_out, const V* v_in, const V* v_alt, V* v_out,
-                             uint32_t n, const uint64_t* __restrict__ red, int aliased) {
-  const int na = __builtin_popcount(varying_bytes(red, (int)sizeof(Bits)));
+                             uint32_t n, const uint64_t* __restrict__ red, int aliased,
+                             const uint32_t* __restrict__ counters, ChainMap cm, uint32_t* __restrict__ status,
+                             const uint32_t* __restrict__ gate) {
+  constexpr int P = (int)sizeof(Bits);
+  if (gated_off(gate)) return;
+  const uint32_t vary = varying_bytes(red, P);
+  if (blockIdx.x == 0 && threadIdx.x < (uint32_t)(P * kChains)) {
+    const uint32_t p = threadIdx.x / kChains, c = threadIdx.x % kChains;
+    if ((vary >> p) & 1u && c < cm.chains) {
+      const uint32_t want = min(cm.tpc, cm.tiles - c * cm.tpc);
+      if (counters[p * kChains + c] < want) atomicOr(status, kStatusChainUnserved);
+    }
+  }
+  const int na = __builtin_popcount(vary);
   int s;
   if (na == 0) {
     s = kBufIn;
@@ -1049,33 +683,47 @@ size_t skew_pair_min() {
   return (size_t)MGS_SKEW_PAIR_MIN;
 }
 
+// Look-back wait bound of the pass kernels (MGS_LB_SPIN_LIMIT sleeps; the
+// environment variable of that name, a decimal count, overrides it, so a
+// test can make every wait give up and check that the status word says so).
+uint32_t lookback_spin_limit() {
+  const char* e = std::getenv("MGS_LB_SPIN_LIMIT");
+  if (e != nullptr && e[0] != 0) {
+    char* end = nullptr;
+    const unsigned long long v = std::strtoull(e, &end, 10);
+    if (end != nullptr && *end == 0) return (uint32_t)std::min<unsigned long long>(v, 0xFFFFFFFFull);
+  }
+  return kLookbackSpinLimit;
+}
+
+// Peer-mask ranking (pass_kernel RANK 3) for 4-byte keys when the
+// environment holds MGS_RANK_PEERS=1.
+bool rank_peers_requested() {
+  const char* e = std::getenv("MGS_RANK_PEERS");
+  return e != nullptr && e[0] == '1' && e[1] == 0;
+}
+
 struct SortLayout {
-  size_t alt_k, alt_v, hist, red, counters, desc, total;
+  size_t status, alt_k, alt_v, hist, red, counters, desc, total;
   uint32_t num_tiles;
   bool wide_desc;
 };
 
-// Keys per look-back unit of a pass: a chunk of MGS_CHUNK_TILES tiles when
-// the tile has the eight wavefronts the chunked kernel needs, else a tile.
-template <int KB, int VB, bool BIG>
-struct PassUnit {
-  using Cfg = PassCfg<KB, VB, BIG>;
-  static constexpr int kChunkTiles = (MGS_CHUNK_TILES > 1 && Cfg::kBlock >= 512) ? MGS_CHUNK_TILES : 1;
-  static constexpr size_t kKeys = (size_t)Cfg::kTile * kChunkTiles;
-};
-
 template <int KB, int VB, bool BIG>
 SortLayout sort_layout_cfg(size_t n) {
-  constexpr size_t unit = PassUnit<KB, VB, BIG>::kKeys;
+  constexpr size_t tile = (size_t)PassCfg<KB, VB, BIG>::kTile;
   SortLayout L{};
-  L.num_tiles = (uint32_t)((n + unit - 1) / unit);
-  L.wide_desc = n > (size_t(1) << 30);
+  L.num_tiles = (uint32_t)((n + tile - 1) / tile);
+  // descriptor values are prefixes inside one chain: 30 bits hold them
+  // unless a chain reaches 2^30 keys
+  L.wide_desc = make_chain_map(n, (uint32_t)tile, kChains).chain_keys >= (size_t(1) << 30);
   size_t o = 0;
   auto take = [&](size_t bytes) {
     const size_t at = o;
     o = align_up(o + bytes);
     return at;
   };
+  L.status = take(sizeof(uint32_t));  // first word of temp: the call's status word
   L.alt_k = take(n * KB);
   L.alt_v = take(n * VB);
   L.hist = take((size_t)KB * kChains * kRadix * 4);
@@ -1089,6 +737,19 @@ SortLayout sort_layout_cfg(size_t n) {
 template <int KB, int VB>
 SortLayout sort_layout(size_t n) {
   return use_big_tiles(KB, VB, n) ? sort_layout_cfg<KB, VB, true>(n) : sort_layout_cfg<KB, VB, false>(n);
+}
+
+// Scratch bytes for n keys: the layout's size, made non-decreasing in n.
+// Wide records switch to larger tiles (fewer descriptors) at kBigTileMin
+// keys, so just above it the small-tile layout of kBigTileMin - 1 keys can
+// be the larger one; a buffer sized for n must serve every smaller sort.
+template <int KB, int VB>
+size_t sort_bytes(size_t n) {
+  const SortLayout L = sort_layout<KB, VB>(n);
+  size_t b = L.total;
+  if (KB == 4 && VB == 0 && msd_eligible(n)) b = std::max(b, msd_layout(L.desc).end);
+  if (KB + VB > 4 && n >= kBigTileMin) b = std::max(b, sort_layout_cfg<KB, VB, false>(kBigTileMin - 1).total);
+  return b;
 }
 
 // Workgroups of `kernel` that fit on the current device at once (persistent
@@ -1115,17 +776,6 @@ uint32_t resident_blocks(K kernel, int block, size_t dyn_lds) {
   return r;
 }
 
-// Records events[i] (when given) before stage i; stage boundaries are the
-// kernel launches of the sequence below.
-struct Marks {
-  hipEvent_t* ev;
-  int cap, used;
-  void operator()(hipStream_t s) {
-    if (ev != nullptr && used < cap) (void)hipEventRecord(ev[used], s);
-    ++used;
-  }
-};
-
 template <int KT, int VT, bool BIG>
 int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, size_t n, void* temp, size_t temp_bytes,
                    hipStream_t s, Marks& mark) {
@@ -1144,15 +794,21 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
   V* vo = static_cast<V*>(vout);
 
   if (n <= (size_t)SCfg::kTile) {
-    mark(s);
+    mark(s, "small_sort");
     small_sort_kernel<KT, V, SCfg::kBlock, SCfg::kIpt><<<1, SCfg::kBlock, 0, s>>>(ki, ko, vi, vo, (uint32_t)n);
     mark(s);
     return check_launch();
   }
 
   const SortLayout L = sort_layout_cfg<KB, VB, BIG>(n);
-  if (temp == nullptr || temp_bytes < L.total) return MGS_ERR_TEMP_TOO_SMALL;
+  // 4-byte keys alone: the three-pass path first (its scratch overlaps the
+  // LSD descriptors, which only a sort it turned itself off for uses)
+  const bool msd = KB == 4 && VB == 0 && msd_eligible(n);
+  const MsdLayout ML = msd_layout(L.desc);
+  const size_t need = msd ? std::max(L.total, ML.end) : L.total;
+  if (temp == nullptr || temp_bytes < need) return MGS_ERR_TEMP_TOO_SMALL;
   char* t = static_cast<char*>(temp);
+  uint32_t* status = reinterpret_cast<uint32_t*>(t + L.status);
   Bits* ka = reinterpret_cast<Bits*>(t + L.alt_k);
   V* va = reinterpret_cast<V*>(reinterpret_cast<VP*>(t + L.alt_v));
   uint32_t* hist = reinterpret_cast<uint32_t*>(t + L.hist);
@@ -1161,13 +817,21 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
   void* desc = t + L.desc;
   const size_t desc_words = (size_t)L.num_tiles * kRadix * (L.wide_desc ? 2 : 1);
   const int aliased = (kin == kout) || (kHasVal<V> && vin == vout);
+  const ChainMap cm = make_chain_map(n, (uint32_t)Cfg::kTile, kChains);
+  const uint32_t spin_limit = lookback_spin_limit();
+  const uint32_t* gate = nullptr;
+  if constexpr (KB == 4 && VB == 0) {
+    if (msd) {
+      const int rc = msd_launch<KT>(kin, kout, ka, n, t, ML, status, s, mark);
+      if (rc != MGS_OK) return rc;
+      gate = reinterpret_cast<const uint32_t*>(t + ML.ctl);  // MsdCtl::off
+    }
+  }
 
-  using PU = PassUnit<KB, VB, BIG>;
-  const ChainMap cm = make_chain_map(n, (uint32_t)PU::kKeys, kChains);
-
-  mark(s);
-  init_kernel<<<1, 1024, 0, s>>>(hist, (uint32_t)(P * kChains * kRadix), counters, (uint32_t)(P * kChains), red);
-  mark(s);
+  mark(s, "init");
+  init_kernel<<<1, 1024, 0, s>>>(hist, (uint32_t)(P * kChains * kRadix), counters, (uint32_t)(P * kChains), red, status,
+                                 gate);
+  mark(s, "upfront");
   {
     constexpr int HB = 256;
     // chain-major split: the same number of workgroups per chain
@@ -1176,24 +840,31 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
     const unsigned grid = per_chain * cm.chains;
     if ((reinterpret_cast<uintptr_t>(kin) % 16) == 0)
       upfront_kernel<KT, HB, true><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words, cm,
-                                                       xcd_local_allowed());
+                                                       xcd_local_allowed(), gate);
     else
       upfront_kernel<KT, HB, false><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words, cm,
-                                                        xcd_local_allowed());
+                                                        xcd_local_allowed(), gate);
   }
   auto launch_pass = [&](auto kernel, auto* dsc, int p) {
     const uint32_t res = resident_blocks(kernel, Cfg::kBlock, 0);
     uint32_t grid = std::min<uint32_t>(L.num_tiles, res ? res : L.num_tiles);
     grid = std::max<uint32_t>(grid, cm.chains);  // every chain has a home workgroup
     kernel<<<grid, Cfg::kBlock, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, p, aliased, cm, red, hist, counters,
-                                        dsc);
+                                        dsc, status, spin_limit, gate);
   };
   // Large sorts launch each pass as a pair of pass_kernel variants, one-
   // atomic-per-key ranking and uniform-digit ranking; the variant that does
   // not suit the pass's digit counts exits at once (a few microseconds).
   const bool rank_pair = BIG && n >= skew_pair_min();
+  const bool peers = KB + VB == 4 && rank_peers_requested();
   auto launch_ranked = [&](auto* dsc, int p) {
     using DT = std::remove_pointer_t<decltype(dsc)>;
+    if constexpr (KB + VB == 4) {
+      if (peers) {
+        launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, DT, Cfg::kWavesPerEu, 3>, dsc, p);
+        return;
+      }
+    }
     if constexpr (BIG) {
       if (rank_pair) {
         launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, DT, Cfg::kWavesPerEu, 1>, dsc, p);
@@ -1203,24 +874,20 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
     }
     launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, DT, Cfg::kWavesPerEu, 0>, dsc, p);
   };
+  static const char* const kPassName[kMaxPasses] = {"pass0", "pass1", "pass2", "pass3",
+                                                    "pass4", "pass5", "pass6", "pass7"};
   for (int p = 0; p < P; ++p) {
-    mark(s);
-    if constexpr (PU::kChunkTiles > 1) {
-      if (L.wide_desc)
-        launch_pass(chunk_pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, PU::kChunkTiles, uint64_t>, (uint64_t*)desc, p);
-      else
-        launch_pass(chunk_pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, PU::kChunkTiles, uint32_t>, (uint32_t*)desc, p);
-    } else {
-      if (L.wide_desc)
-        launch_ranked(static_cast<uint64_t*>(desc), p);
-      else
-        launch_ranked(static_cast<uint32_t*>(desc), p);
-    }
+    mark(s, kPassName[p]);
+    if (L.wide_desc)
+      launch_ranked(static_cast<uint64_t*>(desc), p);
+    else
+      launch_ranked(static_cast<uint32_t*>(desc), p);
   }
-  mark(s);
+  mark(s, "fixup");
   {
     const int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
-    fixup_kernel<Bits, V><<<grid, 256, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, red, aliased);
+    fixup_kernel<Bits, V><<<grid, 256, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, red, aliased, counters, cm, status,
+                                               gate);
   }
   mark(s);
   return check_launch();
@@ -1261,19 +928,54 @@ size_t sort_temp_bytes(mgs_key kt, mgs_val vt, size_t n) {
   else small_tile = SmallCfg<8, 8>::kTile;
   if (n <= small_tile) return 0;
   if (kb == 4) {
-    if (vb == 0) return sort_layout<4, 0>(n).total;
-    if (vb == 4) return sort_layout<4, 4>(n).total;
-    return sort_layout<4, 8>(n).total;
+    if (vb == 0) return sort_bytes<4, 0>(n);
+    if (vb == 4) return sort_bytes<4, 4>(n);
+    return sort_bytes<4, 8>(n);
   }
-  if (vb == 0) return sort_layout<8, 0>(n).total;
-  if (vb == 4) return sort_layout<8, 4>(n).total;
-  return sort_layout<8, 8>(n).total;
+  if (vb == 0) return sort_bytes<8, 0>(n);
+  if (vb == 4) return sort_bytes<8, 4>(n);
+  return sort_bytes<8, 8>(n);
 }
+
+namespace {
+thread_local std::string g_stage_names;
+
+template <int KB, int VB>
+int path_of(size_t n, const void* temp, size_t temp_bytes, hipStream_t s) {
+  using SCfg = SmallCfg<KB, VB>;
+  if (n <= (size_t)SCfg::kTile) return MGS_PATH_SMALL;
+  if (!(KB == 4 && VB == 0 && msd_eligible(n))) return MGS_PATH_LSD;
+  const SortLayout L = sort_layout<KB, VB>(n);
+  const MsdLayout ML = msd_layout(L.desc);
+  if (temp == nullptr || temp_bytes < ML.end) return MGS_ERR_TEMP_TOO_SMALL;
+  uint32_t off = 0;
+  hipError_t e = hipMemcpyAsync(&off, static_cast<const char*>(temp) + ML.ctl, sizeof(off), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return record_hip(e);
+  return off == 0 ? MGS_PATH_THREE_PASS : MGS_PATH_LSD;
+}
+}  // namespace
+
+int sort_path(mgs_key kt, mgs_val vt, size_t n, const void* temp, size_t temp_bytes, hipStream_t s) {
+  const int kb = key_bytes(kt), vb = val_bytes(vt);
+  if (kb <= 0 || vb < 0) return MGS_ERR_BAD_TYPE;
+  if (kb == 4) {
+    if (vb == 0) return path_of<4, 0>(n, temp, temp_bytes, s);
+    if (vb == 4) return path_of<4, 4>(n, temp, temp_bytes, s);
+    return path_of<4, 8>(n, temp, temp_bytes, s);
+  }
+  if (vb == 0) return path_of<8, 0>(n, temp, temp_bytes, s);
+  if (vb == 4) return path_of<8, 4>(n, temp, temp_bytes, s);
+  return path_of<8, 8>(n, temp, temp_bytes, s);
+}
+
+const char* last_stage_names() { return g_stage_names.c_str(); }
 
 int sort_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout, size_t n,
                   void* temp, size_t temp_bytes, hipStream_t s, hipEvent_t* events, int num_events,
                   int* num_marks) {
-  Marks m{events, num_events, 0};
+  g_stage_names.clear();
+  Marks m{events, num_events, 0, num_marks ? &g_stage_names : nullptr};
   int r = MGS_ERR_BAD_TYPE;
   switch (kt) {
     case MGS_KEY_U32: r = sort_by_val<MGS_KEY_U32>(vt, kin, kout, vin, vout, n, temp, temp_bytes, s, m); break;
@@ -1288,9 +990,3 @@ int sort_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const voi
 }
 
 }  // namespace mgs
-
-#ifdef MGS_PHASE_PROBE
-extern "C" int mgs_probe_set(uint64_t* p) {
-  return hipMemcpyToSymbol(HIP_SYMBOL(mgs::g_phase_probe), &p, sizeof(p)) == hipSuccess ? 0 : -1;
-}
-#endif

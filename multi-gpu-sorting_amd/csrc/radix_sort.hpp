@@ -19,6 +19,15 @@ namespace mgs {
 
 constexpr int kMaxPasses = 8;
 
+// Status word: the first 32-bit word of every call's temp buffer, cleared by
+// the call's first kernel.  Kernels set these bits when they detect a
+// condition that leaves the output invalid; mgs_temp_status() reads it.
+enum : uint32_t {
+  kStatusLookback = 1u,        // a look-back wait reached its bound
+  kStatusChainUnserved = 2u,   // a chain's tiles were not all taken
+  kStatusSkipped = 4u,         // segments left unsorted (malformed bounds)
+};
+
 // Buffer ids of the ping-pong plan.
 enum : int32_t { kBufIn = 0, kBufAlt = 1, kBufOut = 2 };
 

@@ -29,6 +29,7 @@ namespace mgs {
 namespace {
 
 struct SegCtl {
+  uint32_t status;  // the call's status word (first word of temp)
   uint32_t n_small, n_medium, n_large, large_tiles;
   uint32_t skipped;  // segments left unsorted: malformed bounds or beyond max_total
 };
@@ -124,6 +125,7 @@ template <bool RANGES>
 __global__ __launch_bounds__(kClassifyBlock) void seg_classify_kernel(const uint64_t* __restrict__ bounds,
                                                                       const uint64_t* __restrict__ val_begin,
                                                                       uint32_t nseg, uint64_t max_total,
+                                                                      uint64_t max_seg,
                                                                       uint32_t small_max, uint32_t medium_max,
                                                                       uint32_t tile, SegTab tab,
                                                                       uint32_t* small_list, uint32_t* medium_list,
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(kClassifyBlock) void seg_classify_kernel(const uint
       const uint64_t lo = RANGES ? bounds[2 * (size_t)s] : bounds[s];
       const uint64_t hi = RANGES ? bounds[2 * (size_t)s + 1] : bounds[s + 1];
       b = lo;
-      ok = hi >= lo && hi - lo < (uint64_t(1) << 32);
+      ok = hi >= lo && hi - lo < (uint64_t(1) << 32) && hi - lo <= max_seg;
       want = ok ? hi - lo : 0;
     }
     // compact scratch slot: the offsets form keeps the segment's distance
@@ -201,6 +203,7 @@ __global__ __launch_bounds__(kClassifyBlock) void seg_classify_kernel(const uint
     ctl->n_large = run_l;
     ctl->large_tiles = run_t;
     ctl->skipped = skipped;
+    ctl->status = skipped ? kStatusSkipped : 0u;
     tile_start[run_l] = run_t;
   }
 }
@@ -459,7 +462,8 @@ __global__ __launch_bounds__(BLOCK) void seg_pass_kernel(const typename Codec<KT
                                                          const uint32_t* __restrict__ tile_start,
                                                          const uint32_t* __restrict__ seg_base,
                                                          uint32_t* __restrict__ tile_counter, D* __restrict__ desc,
-                                                         const SegCtl* __restrict__ ctl, int pass) {
+                                                         const SegCtl* __restrict__ ctl, uint32_t* __restrict__ status,
+                                                         int pass) {
   using Bits = typename Codec<KT>::Bits;
   constexpr int P = (int)sizeof(Bits);
   __shared__ uint32_t s_tile;
@@ -501,7 +505,7 @@ __global__ __launch_bounds__(BLOCK) void seg_pass_kernel(const typename Codec<KT
   }
   const uint32_t t0 = tile_start[li];
   onesweep_tile<KT, V, BLOCK, IPT, D>(src, dst, vsrc, vdst, len, len, g - t0, desc + (size_t)t0 * kRadix,
-                                      seg_base + ((size_t)li * P + pass) * kRadix, pass * 8, pass & 1);
+                                      seg_base + ((size_t)li * P + pass) * kRadix, pass * 8, pass & 1, status);
 }
 
 constexpr size_t kAlign = 256;
@@ -542,20 +546,22 @@ struct SegCfg {
 constexpr uint32_t kMinLargeExclusive = 256;
 
 template <int KB, int VB>
-SegLayout seg_layout(size_t max_total, size_t nseg) {
+SegLayout seg_layout(size_t max_total, size_t nseg, size_t max_seg) {
   using C = SegCfg<KB, VB>;
   constexpr int P = KB;
   SegLayout L{};
   L.max_large = (uint32_t)std::min<size_t>(nseg, max_total / (kMinLargeExclusive + 1) + 1);
   L.tiles_ub = (uint32_t)(max_total / C::T::kTile + L.max_large + 1);
-  L.wide_desc = max_total > (size_t(1) << 30);
+  // descriptor values are prefixes inside one segment: 30 bits hold them
+  // unless a segment may reach 2^30 keys
+  L.wide_desc = max_seg >= (size_t(1) << 30);
   size_t o = 0;
+  L.ctl = o;  // first: its status word is the call's status word
+  o = align_up(o + sizeof(SegCtl));
   L.alt_k = o;
   o = align_up(o + max_total * KB);
   L.alt_v = o;
   o = align_up(o + max_total * VB);
-  L.ctl = o;
-  o = align_up(o + sizeof(SegCtl));
   L.tab_begin = o;
   o = align_up(o + nseg * 8);
   L.tab_vbegin = o;
@@ -583,6 +589,15 @@ SegLayout seg_layout(size_t max_total, size_t nseg) {
   return L;
 }
 
+// Longest segment the call accepts: the caller's hint (mgs_tuning::
+// max_segment_keys) or max_total.  Longer segments are left unsorted and
+// reported in the status word.
+inline size_t segment_bound(size_t max_total, const mgs_tuning* tuning) {
+  if (tuning != nullptr && tuning->max_segment_keys > 0)
+    return std::min<size_t>(max_total, (size_t)tuning->max_segment_keys);
+  return max_total;
+}
+
 template <int KT, int VT>
 int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const uint64_t* bounds, bool ranges,
               const uint64_t* val_begin, size_t nseg, size_t max_total, const mgs_tuning* tuning, void* temp, size_t temp_bytes, hipStream_t s) {
@@ -602,7 +617,8 @@ int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const ui
     if (tuning->medium_max > 0) medium_max = (uint32_t)tuning->medium_max;
   }
   medium_max = std::max({medium_max, small_max, kMinLargeExclusive});
-  const SegLayout L = seg_layout<KB, VB>(max_total, nseg);
+  const size_t max_seg = segment_bound(max_total, tuning);
+  const SegLayout L = seg_layout<KB, VB>(max_total, nseg, max_seg);
   if (temp == nullptr || temp_bytes < L.total) return MGS_ERR_TEMP_TOO_SMALL;
   char* t = static_cast<char*>(temp);
   Bits* ka = reinterpret_cast<Bits*>(t + L.alt_k);
@@ -625,11 +641,11 @@ int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const ui
   V* vo = static_cast<V*>(vout);
 
   if (ranges)
-    seg_classify_kernel<true><<<1, kClassifyBlock, 0, s>>>(bounds, val_begin, (uint32_t)nseg, max_total, small_max, medium_max,
+    seg_classify_kernel<true><<<1, kClassifyBlock, 0, s>>>(bounds, val_begin, (uint32_t)nseg, max_total, max_seg, small_max, medium_max,
                                                            TC::kTile, tab, small_list, medium_list, large_list,
                                                            tile_start, ctl, tcount);
   else
-    seg_classify_kernel<false><<<1, kClassifyBlock, 0, s>>>(bounds, val_begin, (uint32_t)nseg, max_total, small_max, medium_max,
+    seg_classify_kernel<false><<<1, kClassifyBlock, 0, s>>>(bounds, val_begin, (uint32_t)nseg, max_total, max_seg, small_max, medium_max,
                                                             TC::kTile, tab, small_list, medium_list, large_list,
                                                             tile_start, ctl, tcount);
   seg_clear_kernel<<<2048, 256, 0, s>>>(seg_hist, P * kRadix, (uint32_t*)desc, kRadix * (L.wide_desc ? 2 : 1), ctl);
@@ -645,10 +661,10 @@ int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const ui
   for (int p = 0; p < P; ++p) {
     if (L.wide_desc)
       seg_pass_kernel<KT, V, TC::kBlock, TC::kIpt, uint64_t><<<L.tiles_ub, TC::kBlock, 0, s>>>(
-          ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_base, tcount, (uint64_t*)desc, ctl, p);
+          ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_base, tcount, (uint64_t*)desc, ctl, &ctl->status, p);
     else
       seg_pass_kernel<KT, V, TC::kBlock, TC::kIpt, uint32_t><<<L.tiles_ub, TC::kBlock, 0, s>>>(
-          ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_base, tcount, (uint32_t*)desc, ctl, p);
+          ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_base, tcount, (uint32_t*)desc, ctl, &ctl->status, p);
   }
   return check_launch();
 }
@@ -672,23 +688,24 @@ int seg_by_val(mgs_val vt, const void* kin, void* kout, const void* vin, void* v
 }
 
 template <int KB, int VB>
-size_t seg_bytes(size_t max_total, size_t nseg) {
-  return seg_layout<KB, VB>(max_total, nseg).total;
+size_t seg_bytes(size_t max_total, size_t nseg, size_t max_seg) {
+  return seg_layout<KB, VB>(max_total, nseg, max_seg).total;
 }
 
 }  // namespace
 
-size_t segmented_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total, size_t nseg) {
+size_t segmented_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total, size_t nseg, const mgs_tuning* tuning) {
   const int kb = key_bytes(kt), vb = val_bytes(vt);
   if (kb < 0 || vb < 0) return 0;
+  const size_t ms = segment_bound(max_total, tuning);
   if (kb == 4) {
-    if (vb == 0) return seg_bytes<4, 0>(max_total, nseg);
-    if (vb == 4) return seg_bytes<4, 4>(max_total, nseg);
-    return seg_bytes<4, 8>(max_total, nseg);
+    if (vb == 0) return seg_bytes<4, 0>(max_total, nseg, ms);
+    if (vb == 4) return seg_bytes<4, 4>(max_total, nseg, ms);
+    return seg_bytes<4, 8>(max_total, nseg, ms);
   }
-  if (vb == 0) return seg_bytes<8, 0>(max_total, nseg);
-  if (vb == 4) return seg_bytes<8, 4>(max_total, nseg);
-  return seg_bytes<8, 8>(max_total, nseg);
+  if (vb == 0) return seg_bytes<8, 0>(max_total, nseg, ms);
+  if (vb == 4) return seg_bytes<8, 4>(max_total, nseg, ms);
+  return seg_bytes<8, 8>(max_total, nseg, ms);
 }
 
 int segmented_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout,

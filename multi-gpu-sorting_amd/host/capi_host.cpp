@@ -4,16 +4,22 @@
 #include <hip/hip_runtime.h>
 #include <omp.h>
 
+#include <chrono>
 #include <cstring>
+#include <memory>
 #include <parallel/algorithm>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "data_generator.h"
 #include "gowanlock_sort.h"
 #include "host_api.h"
 #include "host_vector.h"
 #include "key_traits.h"
+#include "hip_check.h"
 #include "paradis.h"
+#include "sort_server.h"
 #include "tanasic_sort.h"
 #include "time_durations.h"
 
@@ -129,6 +135,74 @@ int mgh_gowanlock_plan(size_t n, size_t chunk_size, size_t merge_group_size, siz
   out[4] = p.num_chunks_to_merge;
   out[5] = p.final_runs;
   return 0;
+}
+
+int mgh_serve(mgh_serve_request* reqs, size_t count, int gpu, int batched, int max_inflight, int host_data,
+              double budget_gib, double* gkeys_s) {
+  if (reqs == nullptr && count > 0) return -4;
+  return Guard([&] {
+    using mgsort::SortServer;
+    HIP_CHECK(hipSetDevice(gpu));
+    mgsort::ServerConfig cfg;
+    cfg.device = gpu;
+    cfg.batched = batched != 0;
+    cfg.max_inflight = max_inflight > 0 ? max_inflight : 1;
+    cfg.host_data = host_data != 0;
+    cfg.hbm_budget = (size_t)(budget_gib * (double)(size_t(1) << 30));
+    SortServer server(cfg);
+    auto kb = [](int kt) { return (kt == MGS_KEY_U64 || kt == MGS_KEY_I64 || kt == MGS_KEY_F64) ? size_t(8) : size_t(4); };
+    auto vb = [](int vt) { return vt == MGS_VAL_U32 ? size_t(4) : vt == MGS_VAL_U64 ? size_t(8) : size_t(0); };
+    std::vector<std::unique_ptr<mgsort::SortRequest>> rs(count);
+    std::vector<void*> slots(count, nullptr);
+    for (size_t i = 0; i < count; ++i) {
+      const mgh_serve_request& q = reqs[i];
+      auto r = std::make_unique<mgsort::SortRequest>();
+      r->kt = (mgs_key)q.key_type;
+      r->vt = (mgs_val)q.val_type;
+      r->n = q.n;
+      r->keys_in = q.keys_in;
+      r->vals_in = q.vals_in;
+      r->keys_out = q.keys_out;
+      r->vals_out = q.vals_out;
+      if (!host_data) {
+        slots[i] = server.AllocSlot(r->kt, r->vt, q.n);
+        if (!slots[i]) throw mgsort::Error("serving pool too small for the device-mode request set");
+        HIP_CHECK(hipMemcpy(slots[i], q.keys_in, q.n * kb(q.key_type), hipMemcpyHostToDevice));
+        if (vb(q.val_type))
+          HIP_CHECK(hipMemcpy(SortServer::SlotVals(slots[i], r->kt, q.n), q.vals_in, q.n * vb(q.val_type),
+                              hipMemcpyHostToDevice));
+        r->slot = slots[i];
+      }
+      rs[i] = std::move(r);
+    }
+    const double t0 = SortServer::Now();
+    for (size_t i = 0; i < count; ++i) {
+      const double wait = t0 + reqs[i].arrival_s - SortServer::Now();
+      if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+      server.Submit(rs[i].get());
+    }
+    server.Drain();
+    double first = 1e300, last = 0;
+    size_t keys = 0;
+    int rc = 0;
+    for (size_t i = 0; i < count; ++i) {
+      mgsort::SortRequest& r = *rs[i];
+      reqs[i].latency_s = r.t_done - r.t_submit;
+      reqs[i].status = r.status;
+      first = std::min(first, r.t_submit);
+      last = std::max(last, r.t_done);
+      if (r.status == MGS_OK) keys += r.n; else rc = -2;
+      if (!host_data) {
+        HIP_CHECK(hipMemcpy(reqs[i].keys_out, slots[i], r.n * kb(reqs[i].key_type), hipMemcpyDeviceToHost));
+        if (vb(reqs[i].val_type))
+          HIP_CHECK(hipMemcpy(reqs[i].vals_out, SortServer::SlotVals(slots[i], r.kt, r.n),
+                              r.n * vb(reqs[i].val_type), hipMemcpyDeviceToHost));
+        server.FreeSlot(slots[i]);
+      }
+    }
+    if (gkeys_s) *gkeys_s = (count && last > first) ? keys / (last - first) / 1e9 : 0.0;
+    return rc;
+  });
 }
 
 double mgh_phase_seconds(const char* phase) { return mgsort::TimeDurations::Get()->Seconds(phase ? phase : ""); }

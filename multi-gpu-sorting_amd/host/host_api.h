@@ -31,6 +31,32 @@ int mgh_gowanlock_plan(size_t n, size_t chunk_size, size_t merge_group_size, siz
 double mgh_phase_seconds(const char* phase);
 const char* mgh_last_error(void);
 
+// One request of mgh_serve (the serving layer's SortServer, no reference
+// counterpart).  Keys and payloads in host memory; the output arrays may be
+// the input arrays (in place).
+typedef struct mgh_serve_request {
+  int key_type;         // mgs_key
+  int val_type;         // mgs_val
+  size_t n;
+  const void* keys_in;
+  const void* vals_in;  // NULL unless val_type != MGS_VAL_NONE
+  void* keys_out;
+  void* vals_out;
+  double arrival_s;     // submit time, seconds after the first request
+  double latency_s;     // out: submit -> completion event
+  int status;           // out: mgs status of the request's batch
+} mgh_serve_request;
+
+// Runs `count` requests through one SortServer on device `gpu`: batched
+// (continuous batching into segmented launches) or plain (one mgs_sort per
+// request); host_data 1: the server copies host arrays in and out; 0: the
+// requests' data is placed in the server's HBM pool first (as a client that
+// already holds device data) and read back after the run.  *gkeys_s gets the
+// throughput over the run's wall time.  Returns 0 when every request
+// completed with MGS_OK.
+int mgh_serve(mgh_serve_request* reqs, size_t count, int gpu, int batched, int max_inflight, int host_data,
+              double budget_gib, double* gkeys_s);
+
 // hipEvent_t helpers (for mgs_sort_timed).
 void* mgh_event_create(void);
 void mgh_event_destroy(void* e);

@@ -26,12 +26,16 @@ KEY_BYTES = {KEY_U32: 4, KEY_I32: 4, KEY_F32: 4, KEY_U64: 8, KEY_I64: 8, KEY_F64
 VAL_BYTES = {VAL_NONE: 0, VAL_U32: 4, VAL_U64: 8}
 
 MGS_OK = 0
+PATH_SMALL, PATH_LSD, PATH_THREE_PASS = 1, 2, 3
 
 SYMBOLS = [
-    "mgs_sort_temp_bytes", "mgs_sort", "mgs_sort_timed", "mgs_merge_temp_bytes", "mgs_merge", "mgs_select_pivot",
-    "mgs_swap_ranges", "mgs_fill_max", "mgs_segmented_sort_temp_bytes", "mgs_segmented_sort", "mgs_segmented_sort_ranges",
-    "mgs_status_str", "mgs_last_hip_error", "mgs_version",
+    "mgs_sort_temp_bytes", "mgs_sort", "mgs_sort_timed", "mgs_sort_timed_stages", "mgs_sort_path", "mgs_merge_temp_bytes", "mgs_merge", "mgs_select_pivot",
+    "mgs_swap_ranges", "mgs_fill_max", "mgs_segmented_sort_temp_bytes", "mgs_segmented_sort_temp_bytes_tuned",
+    "mgs_segmented_sort", "mgs_segmented_sort_ranges", "mgs_temp_status", "mgs_status_str", "mgs_last_hip_error",
+    "mgs_version",
 ]
+
+MAX_KEYS = 0xFFFF0000  # MGS_MAX_KEYS
 
 
 class MgsError(RuntimeError):
@@ -49,6 +53,10 @@ def _load() -> ctypes.CDLL:
     lib.mgs_sort.restype = i
     lib.mgs_sort_timed.argtypes = [i, i, vp, vp, vp, vp, sz, vp, sz, vp, ctypes.POINTER(vp), i, ctypes.POINTER(i)]
     lib.mgs_sort_timed.restype = i
+    lib.mgs_sort_path.argtypes = [i, i, sz, vp, sz, vp]
+    lib.mgs_sort_path.restype = i
+    lib.mgs_sort_timed_stages.argtypes = []
+    lib.mgs_sort_timed_stages.restype = ctypes.c_char_p
     lib.mgs_merge_temp_bytes.argtypes = [i, i, sz, sz]
     lib.mgs_merge_temp_bytes.restype = sz
     lib.mgs_merge.argtypes = [i, i, vp, vp, sz, vp, vp, sz, vp, vp, vp, sz, vp]
@@ -61,6 +69,10 @@ def _load() -> ctypes.CDLL:
     lib.mgs_fill_max.restype = i
     lib.mgs_segmented_sort_temp_bytes.argtypes = [i, i, sz, sz]
     lib.mgs_segmented_sort_temp_bytes.restype = sz
+    lib.mgs_segmented_sort_temp_bytes_tuned.argtypes = [i, i, sz, sz, vp]
+    lib.mgs_segmented_sort_temp_bytes_tuned.restype = sz
+    lib.mgs_temp_status.argtypes = [vp, sz, vp]
+    lib.mgs_temp_status.restype = i
     lib.mgs_segmented_sort.argtypes = [i, i, vp, vp, vp, vp, vp, sz, sz, vp, vp, sz, vp]
     lib.mgs_segmented_sort.restype = i
     lib.mgs_segmented_sort_ranges.argtypes = [i, i, vp, vp, vp, vp, sz, sz, vp, vp, sz, vp]
@@ -78,8 +90,8 @@ lib = _load()
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("tile_keys", ctypes.c_int32), ("small_max", ctypes.c_int32),
-                ("medium_max", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+    _fields_ = [("large_variant", ctypes.c_int32), ("small_max", ctypes.c_int32),
+                ("medium_max", ctypes.c_int32), ("max_segment_keys", ctypes.c_int32)]
 
 
 def _check(st: int, what: str) -> None:
@@ -105,8 +117,26 @@ def val_type(vt) -> int:
     return VAL_NAMES[vt] if (vt is None or isinstance(vt, str)) else int(vt)
 
 
-def temp_buffer(nbytes: int, device) -> torch.Tensor:
-    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+def temp_buffer(nbytes: int, device, stream=None) -> torch.Tensor:
+    """Scratch for one call.  With `stream`, the buffer belongs to that
+    stream in torch's caching allocator, so it is not handed to other work
+    before the call's kernels on `stream` have finished with it."""
+    if stream is None:
+        return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+    with torch.cuda.stream(stream):
+        return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def temp_status(temp: Optional[torch.Tensor], stream=None) -> int:
+    """mgs_temp_status: waits for `stream` and returns the device-side status
+    (MGS_OK, MGS_ERR_DEVICE, MGS_ERR_SKIPPED) of the last call that used temp."""
+    if temp is None:
+        return MGS_OK
+    return lib.mgs_temp_status(_ptr(temp), temp.numel(), _stream(stream))
+
+
+def check_temp(temp: Optional[torch.Tensor], what: str, stream=None) -> None:
+    _check(temp_status(temp, stream), what)
 
 
 def sort(kt, keys: torch.Tensor, vals: Optional[torch.Tensor] = None, keys_out: Optional[torch.Tensor] = None,
@@ -121,7 +151,7 @@ def sort(kt, keys: torch.Tensor, vals: Optional[torch.Tensor] = None, keys_out: 
         vals_out = vals
     need = lib.mgs_sort_temp_bytes(kt, vt, n)
     if temp is None:
-        temp = temp_buffer(need, keys.device)
+        temp = temp_buffer(need, keys.device, stream)
     st = lib.mgs_sort(kt, vt, _ptr(keys), _ptr(keys_out), _ptr(vals), _ptr(vals_out), n, _ptr(temp),
                       temp.numel(), _stream(stream))
     _check(st, "mgs_sort")
@@ -133,7 +163,7 @@ def merge(kt, a: torch.Tensor, b: torch.Tensor, out: torch.Tensor, av=None, bv=N
     vt = VAL_NONE if av is None else (VAL_U32 if av.element_size() == 4 else VAL_U64)
     need = lib.mgs_merge_temp_bytes(kt, vt, a.numel(), b.numel())
     if temp is None:
-        temp = temp_buffer(need, out.device)
+        temp = temp_buffer(need, out.device, stream)
     st = lib.mgs_merge(kt, vt, _ptr(a), _ptr(av), a.numel(), _ptr(b), _ptr(bv), b.numel(), _ptr(out),
                        _ptr(out_vals), _ptr(temp), temp.numel(), _stream(stream))
     _check(st, "mgs_merge")
@@ -175,10 +205,10 @@ def segmented_sort(kt, keys: torch.Tensor, offsets: torch.Tensor, vals: Optional
         keys_out = keys
     if vals is not None and vals_out is None:
         vals_out = vals
-    need = lib.mgs_segmented_sort_temp_bytes(kt, vt, max_total, nseg)
-    if temp is None:
-        temp = temp_buffer(need, keys.device)
     tp = ctypes.byref(tuning) if tuning is not None else None
+    need = lib.mgs_segmented_sort_temp_bytes_tuned(kt, vt, max_total, nseg, tp)
+    if temp is None:
+        temp = temp_buffer(need, keys.device, stream)
     st = lib.mgs_segmented_sort(kt, vt, _ptr(keys), _ptr(keys_out), _ptr(vals), _ptr(vals_out), _ptr(offsets),
                                 nseg, max_total, tp, _ptr(temp), temp.numel(), _stream(stream))
     _check(st, "mgs_segmented_sort")
@@ -195,10 +225,10 @@ def segmented_sort_ranges(kt, keys: torch.Tensor, ranges: torch.Tensor, vals: Op
     nseg = ranges.numel() // 2
     if max_total is None:
         max_total = keys.numel()
-    need = lib.mgs_segmented_sort_temp_bytes(kt, vt, max_total, nseg)
-    if temp is None:
-        temp = temp_buffer(need, keys.device)
     tp = ctypes.byref(tuning) if tuning is not None else None
+    need = lib.mgs_segmented_sort_temp_bytes_tuned(kt, vt, max_total, nseg, tp)
+    if temp is None:
+        temp = temp_buffer(need, keys.device, stream)
     st = lib.mgs_segmented_sort_ranges(kt, vt, _ptr(keys), _ptr(vals), _ptr(ranges), _ptr(val_begin), nseg, max_total, tp,
                                        _ptr(temp), temp.numel(), _stream(stream))
     _check(st, "mgs_segmented_sort_ranges")

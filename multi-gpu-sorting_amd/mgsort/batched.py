@@ -140,7 +140,7 @@ def autotune(keys: torch.Tensor, vals: torch.Tensor, d_off: torch.Tensor, out_k,
     """Times every candidate tuning on the workload itself; returns the fastest."""
     best, best_t = None, math.inf
     for c in candidates:
-        t = Tuning(c.get("tile_keys", 0), c.get("small_max", 0), c.get("medium_max", 0), 0)
+        t = Tuning(c.get("large_variant", 0), c.get("small_max", 0), c.get("medium_max", 0), c.get("max_segment_keys", 0))
         segmented_sort(KEY_U64, keys, d_off, vals, keys_out=out_k, vals_out=out_v, tuning=t, temp=temp,
                        stream=stream)
         torch.cuda.synchronize()
@@ -182,7 +182,7 @@ def run(dist: str = "uniform", nseg: int = 4096, seed: int = 2024, reps: int = 3
     temp = temp_buffer(need, dev)
     with torch.cuda.stream(stream):
         tuned = autotune(keys, vals, d_off, out_k, out_v, temp, stream, candidates)
-    tn = Tuning(tuned.get("tile_keys", 0), tuned.get("small_max", 0), tuned.get("medium_max", 0), 0)
+    tn = Tuning(tuned.get("large_variant", 0), tuned.get("small_max", 0), tuned.get("medium_max", 0), tuned.get("max_segment_keys", 0))
 
     # the whole batch as one graph, replayed
     g = torch.cuda.CUDAGraph()

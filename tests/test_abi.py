@@ -51,7 +51,7 @@ def test_status_strings_and_version():
     import mgsort
     L = mgsort.lib
     assert L.mgs_status_str(0) == b"ok"
-    for st in (-1, -2, -3, -4, -5):
+    for st in (-1, -2, -3, -4, -5, -6, -7):
         assert L.mgs_status_str(st) and L.mgs_status_str(st) != b"ok"
     assert L.mgs_version()
 
@@ -65,6 +65,46 @@ def test_temp_bytes_monotone_and_typed():
     assert 0 < a < b < c
     # the out-of-place sort needs at least one alternate key buffer
     assert b >= 4 << 24
+
+
+PAIRS = [(k, v) for k in range(6) for v in range(3)]
+
+
+@pytest.mark.parametrize("kt,vt", PAIRS)
+def test_sort_temp_bytes_never_decrease(kt, vt):
+    """A buffer sized for n serves every smaller sort: the scratch query does
+    not decrease across the small-sort limit, the switch to large tiles at
+    2^21 keys for records wider than 4 bytes, and up to MGS_MAX_KEYS."""
+    import mgsort
+    L = mgsort.lib
+    ns = sorted({1, 2047, 2048, 4095, 4096, 4097, 8191, 8192, 8193, (1 << 21) - 10000, (1 << 21) - 1, 1 << 21,
+                 (1 << 21) + 1, (1 << 21) + 10000, 1 << 24, (1 << 26) - 1, 1 << 26, 1 << 30, (1 << 30) + 1,
+                 mgsort.MAX_KEYS - 1, mgsort.MAX_KEYS})
+    sizes = [L.mgs_sort_temp_bytes(kt, vt, n) for n in ns]
+    assert all(b >= a for a, b in zip(sizes, sizes[1:])), list(zip(ns, sizes))
+    # the alternate buffers are in the scratch of every sort that needs scratch
+    kb, vb = mgsort.KEY_BYTES[kt], mgsort.VAL_BYTES[vt]
+    for n, b in zip(ns, sizes):
+        assert b == 0 or b >= n * (kb + vb)
+
+
+def test_too_many_keys_is_rejected_before_any_device_work():
+    """n above MGS_MAX_KEYS (the last tile must end below 2^32: the kernels
+    index keys with 32 bits) returns MGS_ERR_TOO_LARGE before touching the
+    (bogus) pointers; MGS_MAX_KEYS itself passes the size check and fails on
+    the NULL pointer."""
+    import mgsort
+    L = mgsort.lib
+    bogus = ctypes.c_void_p(0x1000)
+    for n in (mgsort.MAX_KEYS + 1, (1 << 32) - 1, 1 << 32, 1 << 40):
+        assert L.mgs_sort(0, 0, bogus, bogus, None, None, n, bogus, 1 << 40, None) == -5
+        assert L.mgs_segmented_sort(0, 0, bogus, bogus, None, None, bogus, 1, n, None, bogus, 1 << 40, None) == -5
+    assert L.mgs_sort(0, 0, None, None, None, None, mgsort.MAX_KEYS, None, 0, None) == -4
+
+
+def test_temp_status_without_scratch_is_ok():
+    import mgsort
+    assert mgsort.lib.mgs_temp_status(None, 0, None) == 0
 
 
 def test_argument_checks_need_no_device():

@@ -136,8 +136,10 @@ def test_segmented_malformed_offsets_are_skipped(mg):
     # earlier offset -> skipped, [30000,50000) ok, [50000,60000) beyond max_total
     off = np.array([0, 10_000, 5_000, 30_000, 50_000, 60_000], dtype=np.uint64)
     din = to_dev(buf)
-    mg.segmented_sort("u32", din, torch.from_numpy(off.view(np.int64)).cuda(), max_total=50_000)
-    torch.cuda.synchronize()
+    temp = mg.temp_buffer(mg.lib.mgs_segmented_sort_temp_bytes(mg.KEY_U32, mg.VAL_NONE, 50_000, 5), "cuda")
+    mg.segmented_sort("u32", din, torch.from_numpy(off.view(np.int64)).cuda(), max_total=50_000, temp=temp)
+    # the call reports the skipped segments in its status word
+    assert mg.temp_status(temp) == -7
     got = from_dev(din, np.uint32)
     exp = buf.copy()
     for a, b in [(0, 10_000), (30_000, 50_000)]:
@@ -175,3 +177,33 @@ def test_segmented_ranges_separate_payload_places(mg):
     torch.cuda.synchronize()
     assert np.array_equal(from_dev(dk, np.uint64), ek)
     assert np.array_equal(from_dev(dv, np.uint32), ev)
+
+
+def test_segmented_max_segment_hint(mg):
+    """mgs_tuning.max_segment_keys: segments up to the hint sort (with the
+    smaller scratch of 4-byte descriptors); a longer one is left unsorted and
+    reported as skipped; a well-formed call reports ok."""
+    import ctypes
+    import torch
+    rng = np.random.default_rng(24)
+    lengths = np.array([1000, 300_000, 50_000, 0, 7])
+    n = int(lengths.sum())
+    keys = random_keys(O.K_U64, n, rng)
+    vals = np.arange(n, dtype=np.uint32)
+    off = make_offsets(lengths)
+    doff = torch.from_numpy(off.view(np.int64)).cuda()
+    for hint, skipped in ((300_000, []), (100_000, [1])):
+        tn = mg.Tuning(0, 0, 0, hint)
+        need = mg.lib.mgs_segmented_sort_temp_bytes_tuned(mg.KEY_U64, mg.VAL_U32, n, len(lengths), ctypes.byref(tn))
+        assert need <= mg.lib.mgs_segmented_sort_temp_bytes(mg.KEY_U64, mg.VAL_U32, n, len(lengths))
+        temp = mg.temp_buffer(need, "cuda")
+        dk, dv = to_dev(keys), to_dev(vals)
+        mg.segmented_sort("u64", dk, doff, dv, tuning=tn, temp=temp)
+        assert mg.temp_status(temp) == (-7 if skipped else 0)
+        ek, ev = keys.copy(), vals.copy()
+        for s in range(len(lengths)):
+            a, b = int(off[s]), int(off[s + 1])
+            if s not in skipped:
+                ek[a:b], ev[a:b] = O.sort(O.K_U64, keys[a:b], vals[a:b])
+        assert np.array_equal(from_dev(dk, np.uint64), ek), hint
+        assert np.array_equal(from_dev(dv, np.uint32), ev), hint

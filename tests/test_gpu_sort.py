@@ -197,3 +197,106 @@ def test_sort_large_tile_geometry(mg, kt, vdt):
     assert np.array_equal(rk, ek)
     if vdt is not None:
         assert np.array_equal(rv, ev)
+
+
+@pytest.mark.parametrize("kt", [O.K_U32, O.K_F32, O.K_I32], ids=lambda k: KEY_NAME[k])
+@pytest.mark.parametrize("n", [100_003, (1 << 21) + 999])
+@pytest.mark.parametrize("dist", ["uniform", "few", "sorted"])
+def test_rank_peers_variant(mg, kt, n, dist, monkeypatch):
+    """MGS_RANK_PEERS=1 ranks digits with peer masks (lanes with equal
+    digits find each other through an LDS OR; stability relies only on the
+    in-order execution of a wavefront's DS instructions) instead of one
+    ds_add_rtn_u32 per key: same result as the oracle."""
+    import torch
+    rng = np.random.default_rng(n + kt)
+    keys = random_keys(kt, n, rng, dist)
+    monkeypatch.setenv("MGS_RANK_PEERS", "1")
+    dk = to_dev(keys)
+    temp = mg.temp_buffer(mg.lib.mgs_sort_temp_bytes(kt, mg.VAL_NONE, n), "cuda")
+    mg.sort(KEY_NAME[kt], dk, temp=temp)
+    assert mg.temp_status(temp) == 0
+    assert np.array_equal(from_dev(dk, keys.dtype), O.sort(kt, keys)[0])
+
+
+@pytest.mark.parametrize("kt,vdt", [(O.K_U32, None), (O.K_U64, np.uint32)], ids=["u32", "u64+v32"])
+def test_lookback_bound_reports_status(mg, kt, vdt, monkeypatch):
+    """The look-back waits are bounded.  With the bound at 0 sleeps
+    (MGS_LB_SPIN_LIMIT=0) a tile whose predecessor has not published yet gives
+    up at once; every kernel still ends, and the call's status word says
+    MGS_ERR_DEVICE.  A call whose status is ok must have the oracle's output;
+    with the default bound the status is ok."""
+    import torch
+    rng = np.random.default_rng(5 + kt)
+    n = 1 << 24
+    keys = random_keys(kt, n, rng)
+    vals = None if vdt is None else random_vals(vdt, n, rng)
+    ek, ev = O.sort(kt, keys, vals)
+    vt = mg.VAL_NONE if vdt is None else mg.VAL_U32
+    temp = mg.temp_buffer(mg.lib.mgs_sort_temp_bytes(kt, vt, n), "cuda")
+    seen = []
+    for limit in ("0", "0", None):
+        if limit is None:
+            monkeypatch.delenv("MGS_LB_SPIN_LIMIT", raising=False)
+        else:
+            monkeypatch.setenv("MGS_LB_SPIN_LIMIT", limit)
+        dk, dv = to_dev(keys), to_dev(vals)
+        mg.sort(KEY_NAME[kt], dk, dv, temp=temp)
+        st = mg.temp_status(temp)
+        seen.append(st)
+        assert st in (0, -6)
+        if st == 0:
+            assert np.array_equal(from_dev(dk, keys.dtype), ek), limit
+            if vdt is not None:
+                assert np.array_equal(from_dev(dv, vals.dtype), ev), limit
+    assert seen[-1] == 0
+
+
+MSD_KEYS = [O.K_U32, O.K_I32, O.K_F32]
+
+
+@pytest.mark.parametrize("kt", MSD_KEYS, ids=lambda k: KEY_NAME[k])
+@pytest.mark.parametrize("n", [4097, 100_003, 1 << 20, (1 << 21) + 777])
+@pytest.mark.parametrize("dist,path", [("uniform", 3), ("sorted", 3), ("reverse", 3), ("few", 2), ("equal", 2),
+                                       ("lowbits", 2)])
+def test_three_pass_path(mg, kt, n, dist, path, monkeypatch):
+    """4-byte keys alone try the three-pass path (MSD pass on byte 3, pass on
+    byte 2 inside each bucket, LDS sort of each batch of 16-bit sub-buckets);
+    MGS_MSD_MIN=0 tries it at any size.  Keys whose top 16 bits crowd into
+    a few sub-buckets (few, equal, lowbits) hand over to the LSD passes.
+    Either way the output is the oracle's, in place and out of place."""
+    import torch
+    monkeypatch.setenv("MGS_MSD_MIN", "0")
+    rng = np.random.default_rng(n + 3 * kt)
+    keys = random_keys(kt, n, rng, dist)
+    ek, _ = O.sort(kt, keys)
+    need = mg.lib.mgs_sort_temp_bytes(kt, mg.VAL_NONE, n)
+    temp = mg.temp_buffer(need, "cuda")
+    for inplace in (True, False):
+        dk = to_dev(keys)
+        out = dk if inplace else torch.empty_like(dk)
+        mg.sort(KEY_NAME[kt], dk, keys_out=out, temp=temp)
+        assert mg.temp_status(temp) == 0
+        got = mg.lib.mgs_sort_path(kt, mg.VAL_NONE, n, mg._ptr(temp), temp.numel(), None)
+        small = n <= 4096
+        assert got == (mg.PATH_SMALL if small else path), (inplace, got)
+        assert np.array_equal(from_dev(out, keys.dtype), ek), inplace
+        if not inplace:
+            assert np.array_equal(from_dev(dk, keys.dtype), keys)
+
+
+@pytest.mark.parametrize("dist", ["uniform", "sorted", "skewed"])
+def test_three_pass_default_size(mg, dist):
+    """From 2^26 keys the path is on by default (no environment)."""
+    import torch
+    kt = O.K_U32
+    n = (1 << 26) + 12_345
+    rng = np.random.default_rng(26)
+    keys = random_keys(kt, n, rng, dist)
+    ek, _ = O.sort(kt, keys)
+    temp = mg.temp_buffer(mg.lib.mgs_sort_temp_bytes(kt, mg.VAL_NONE, n), "cuda")
+    dk = to_dev(keys)
+    mg.sort("u32", dk, temp=temp)
+    assert mg.temp_status(temp) == 0
+    path = mg.lib.mgs_sort_path(kt, mg.VAL_NONE, n, mg._ptr(temp), temp.numel(), None)
+    assert path == (mg.PATH_LSD if dist == "skewed" else mg.PATH_THREE_PASS)
+    assert np.array_equal(from_dev(dk, np.uint32), ek)
