@@ -66,6 +66,8 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
                                                                uint32_t* __restrict__ slices,
                                                                uint32_t* __restrict__ chain_hist,
                                                                uint32_t* __restrict__ chain_bad,
+                                                               uint32_t* __restrict__ xcd_of,
+                                                               uint32_t* __restrict__ desc, uint32_t desc_tiles,
                                                                uint32_t* __restrict__ status) {
   // 65536 16-bit counters, two per word: bin b in word b >> 1, half b & 1
   __shared__ uint32_t h[kMsdBins / 2];
@@ -75,8 +77,25 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
   if (c == 0 && tid == 0) {  // nothing else writes these during this kernel
     *status = 0;
     ctl->off = 0;
+    ctl->xcd_mask = 0;
   }
   for (int i = tid; i < kMsdBins / 2; i += kMsdBlock) h[i] = 0;
+  if (tid == 0) xcd_of[c] = xcc_id();
+  {
+    // look-back descriptors of both passes: the tiles of the input read as
+    // "empty" by pass 1 (status rotation 0) and, once pass 1 has left them
+    // inclusive, by pass 2 (rotation 1); the up to 256 further tiles of
+    // pass 2 (one partial tile per bucket) start as "empty" under rotation 1
+    uint4* d4 = reinterpret_cast<uint4*>(desc);
+    const size_t words = (size_t)desc_tiles * kRadix;
+    const size_t total = words + (size_t)(kRadix + kXcds) * kRadix;
+    const size_t g = (size_t)c * kMsdBlock + tid, st = (size_t)gridDim.x * kMsdBlock;
+    const uint32_t empty1 = Desc<uint32_t>::make(kStEmpty, 0u, 1);
+    for (size_t i = g; i < total / 4; i += st) {
+      const uint32_t v = 4 * i < words ? 0u : empty1;
+      d4[i] = make_uint4(v, v, v, v);
+    }
+  }
   __syncthreads();
   const size_t lo = (size_t)c * chain_keys;
   const size_t hi = std::min<size_t>((size_t)n, lo + chain_keys);
@@ -151,14 +170,20 @@ __global__ __launch_bounds__(256) void msd_scan_kernel(const uint32_t* __restric
                                                        uint32_t* __restrict__ bucket_tot,
                                                        uint32_t* __restrict__ leaders,
                                                        uint32_t* __restrict__ batch_cnt,
-                                                       const uint32_t* __restrict__ chain_bad) {
+                                                       const uint32_t* __restrict__ chain_bad,
+                                                       const uint32_t* __restrict__ xcd_of) {
   __shared__ uint32_t sums[4];
   __shared__ uint32_t s_carry;
   const uint32_t b = blockIdx.x, d = threadIdx.x;
   const uint32_t s = b * kRadix + d;
-  if (b == 0)
-    for (uint32_t c = d; c < chains; c += kRadix)
+  if (b == 0) {
+    uint32_t m = 0;
+    for (uint32_t c = d; c < chains; c += kRadix) {
       if (chain_bad[c]) atomicOr(&ctl->off, kOffOverflow);
+      m |= 1u << (xcd_of[c] & 31u);
+    }
+    atomicOr(&ctl->xcd_mask, m);
+  }
   // sub-bucket size: the chains' 16-bit counters of bin s
   uint32_t cnt = 0;
   {
@@ -225,10 +250,11 @@ __global__ __launch_bounds__(256) void msd_finish_kernel(MsdCtl* __restrict__ ct
                                                          const uint32_t* __restrict__ bucket_tot,
                                                          uint32_t* __restrict__ bucket_start,
                                                          const uint32_t* __restrict__ batch_cnt,
-                                                         uint32_t* __restrict__ batch_base, uint32_t n,
-                                                         uint32_t chain_keys, uint32_t chains,
-                                                         uint64_t* __restrict__ ctr1, uint64_t* __restrict__ ctr2,
-                                                         uint32_t* __restrict__ claim1, uint32_t* __restrict__ claim2) {
+                                                         uint32_t* __restrict__ batch_base,
+                                                         uint32_t chains1, uint32_t tpc1,
+                                                         uint32_t* __restrict__ tile_base1,
+                                                         uint32_t* __restrict__ tile_base2,
+                                                         uint32_t* __restrict__ counters) {
   __shared__ uint32_t sums[4];
   const uint32_t d = threadIdx.x;
   const uint32_t bt = bucket_tot[d];
@@ -244,66 +270,54 @@ __global__ __launch_bounds__(256) void msd_finish_kernel(MsdCtl* __restrict__ ct
     ctl->batches = bb + bc;
     ctl->batch_next = 0;
   }
-  // tile counters: front index in the low word, back index (the tile count)
-  // in the high word
-  ctr2[d] = (uint64_t)((bt + kMsdTile - 1) / kMsdTile) << 32;
-  claim2[d] = 0;
-  for (uint32_t c = d; c < chains; c += kRadix) {
-    const size_t lo = (size_t)c * chain_keys;
-    const uint32_t len = lo < n ? (uint32_t)std::min<size_t>(chain_keys, n - lo) : 0u;
-    ctr1[c] = (uint64_t)((len + kMsdTile - 1) / kMsdTile) << 32;
-    claim1[c] = 0;
-  }
+  __syncthreads();
+  // descriptor tiles: pass 1 chain C from C * tpc1; pass 2 bucket b after
+  // the tiles of the buckets before it
+  const uint32_t nt = (bt + kMsdTile - 1) / kMsdTile;
+  tile_base2[d] = block_excl_scan_256(nt, sums, (int)d);
+  if (d < chains1) tile_base1[d] = d * tpc1;
+  counters[d] = 0;  // pass 2
+  if (d < kXcds) counters[kRadix + d] = 0;  // pass 1
 }
 
 // ---------------------------------------------------------------------------
-// passes 1 and 2
+// passes 1 and 2: onesweep passes over chains of tiles with decoupled
+// look-back inside each chain (the look-back of the LSD pass kernel).
+//   pass 1  8 chains: equal ranges of the input; the digit bases of chain C
+//           are the byte-3 bucket starts plus the keys of that digit in
+//           the ranges before it (the upfront's per-range histograms)
+//   pass 2  256 chains: the byte-3 buckets; digit bases = sub-bucket starts
+// Chain c is served by the workgroups of XCD c mod 8 (when the upfront ran
+// on all eight XCDs), which start together on their XCD's first chain and
+// move on together, so at any time the 32 workgroups of an XCD work on
+// consecutive tiles of one chain: every digit run of the chain's output is
+// then written by consecutive tiles at about the same time.
 // ---------------------------------------------------------------------------
 struct MsdPassArgs {
   const uint32_t* src;
   uint32_t* dst;
   uint32_t n;
-  uint32_t chains;      // pass 1: input chains; pass 2: 256 buckets
-  uint32_t chain_keys;  // pass 1: keys per chain
-  int shift;            // 24 or 16
+  uint32_t chains;             // 8 (pass 1) or 256 (pass 2)
   const uint32_t* bucket_start;
-  const uint32_t* chain_hist;  // pass 1 digit totals per chain
-  const uint32_t* chain_pref;  // pass 1 prefix over chains per digit
-  const uint32_t* sub_cnt;     // pass 2 digit totals per bucket
-  const uint32_t* sub_off;     // pass 2 digit offsets inside each bucket
-  uint64_t* ctr;
-  uint32_t* claim;
+  const uint32_t* chain_pref;  // pass 1: prefix over the upfront's ranges per digit
+  uint32_t ranges_per_chain;   // pass 1: upfront ranges per chain
+  uint32_t chain_keys;         // pass 1: keys per chain
+  const uint32_t* sub_off;     // pass 2: sub-bucket offsets inside each bucket
+  const uint32_t* tile_base;   // first descriptor tile of each chain
+  uint32_t* counters;          // tiles taken per chain
+  uint32_t* desc;              // look-back descriptors, 256 per tile
+  int shift;                   // 24 or 16
+  int parity;                  // descriptor status rotation of this pass
   const MsdCtl* ctl;
+  uint32_t* status;
+  uint32_t spin_limit;
 };
 
-// Takes the next tile of chain c from the front or the back end.
-__device__ __forceinline__ uint32_t msd_take(uint64_t* ctr, uint32_t c, bool back) {
-  const uint64_t old = atomicAdd(reinterpret_cast<unsigned long long*>(&ctr[c]),
-                                 back ? 0xFFFFFFFF00000000ull : 1ull);
-  const uint32_t f = (uint32_t)old, b = (uint32_t)(old >> 32);
-  if (f >= b) return kNoWork;
-  return back ? b - 1 : f;
-}
-
-// Claims the back end of the first chain, from `start` on (cyclically), that
-// has tiles left and no back-end worker yet, and takes its last free tile.
-__device__ __forceinline__ void msd_find_back(uint64_t* ctr, uint32_t* claim, uint32_t chains, uint32_t start,
-                                              uint32_t& chain, uint32_t& tile) {
-  chain = kNoWork;
-  tile = kNoWork;
-  for (uint32_t i = 0; i < chains; ++i) {
-    const uint32_t c = (start + i) % chains;
-    const uint64_t v = __hip_atomic_load(&ctr[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((uint32_t)v >= (uint32_t)(v >> 32)) continue;
-    if (__hip_atomic_load(&claim[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) continue;
-    if (atomicCAS(&claim[c], 0u, 1u) != 0u) continue;
-    const uint32_t t = msd_take(ctr, c, true);
-    if (t != kNoWork) {
-      chain = c;
-      tile = t;
-      return;
-    }
-  }
+__device__ __forceinline__ void msd_publish(uint32_t* p, uint32_t v, bool xcd_local) {
+  if (xcd_local)
+    st_wg(p, v);
+  else
+    st_agent(p, v);
 }
 
 template <int KT, int PASS>
@@ -311,21 +325,23 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
   constexpr int W = kMsdBlock / kWave;
   constexpr int IPT = kMsdIpt;
   constexpr int TILE = kMsdTile;
+  using D = uint32_t;
   if (a.ctl->off) return;
 
   __shared__ uint32_t keys[TILE];
   __shared__ uint32_t off[W][kRadix];
-  __shared__ uint32_t run[kRadix];
   __shared__ uint32_t gbase[kRadix];
+  __shared__ uint32_t s_prefix[kRadix];
   __shared__ uint32_t sums[4];
-  __shared__ uint32_t s_tile, s_chain, s_back;
+  __shared__ uint32_t s_tile, s_chain;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int w = tid / kWave;
+  const bool lb_wave = tid >= kRadix && tid < 2 * kRadix;
+  const int taker = kRadix;
   for (int i = tid; i < W * kRadix; i += kMsdBlock) (&off[0][0])[i] = 0;
 
-  // chain bounds and digit bases
   auto chain_lo = [&](uint32_t c) -> uint32_t {
     if constexpr (PASS == 1) return c * a.chain_keys;
     else return a.bucket_start[c];
@@ -334,44 +350,36 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
     if constexpr (PASS == 1) return (uint32_t)std::min<size_t>((size_t)(c + 1) * a.chain_keys, a.n);
     else return a.bucket_start[c + 1];
   };
-  // thread d < 256: first output position of digit d of chain c (front) or
-  // one past its last (back)
-  auto digit_start = [&](uint32_t c, uint32_t d, bool back) -> uint32_t {
-    uint32_t base, tot;
-    if constexpr (PASS == 1) {
-      base = a.bucket_start[d] + a.chain_pref[(size_t)c * kRadix + d];
-      tot = a.chain_hist[(size_t)c * kRadix + d];
-    } else {
-      base = a.bucket_start[c] + a.sub_off[c * kRadix + d];
-      tot = a.sub_cnt[c * kRadix + d];
-    }
-    return back ? base + tot : base;
+  // thread d < 256: output position of the chain's first key of digit d
+  auto digit_base = [&](uint32_t c, uint32_t d) -> uint32_t {
+    if constexpr (PASS == 1) return a.bucket_start[d] + a.chain_pref[(size_t)c * a.ranges_per_chain * kRadix + d];
+    else return a.bucket_start[c] + a.sub_off[c * kRadix + d];
   };
 
-  // the workgroup's first unit of work: its own chain from the front, else
-  // the back end of another chain
-  if (tid == 0) {
-    uint32_t c = kNoWork, t = kNoWork, bk = 0;
-    if (blockIdx.x < a.chains) {
-      c = blockIdx.x;
-      t = msd_take(a.ctr, c, false);
-    }
-    if (t == kNoWork) {
-      msd_find_back(a.ctr, a.claim, a.chains, blockIdx.x, c, t);
-      bk = 1;
-    }
-    s_chain = c;
-    s_tile = t;
-    s_back = bk;
+  const bool xcd_local = a.chains % kXcds == 0 && a.ctl->xcd_mask == (1u << kXcds) - 1;
+  uint32_t my_chain = blockIdx.x % a.chains, chains_left = a.chains, chain_step = 1;
+  if (xcd_local) {
+    my_chain = xcc_id();
+    chains_left = a.chains / kXcds;
+    chain_step = kXcds;
   }
-  __syncthreads();
-  if (s_tile == kNoWork) return;
-  if (tid < kRadix) run[tid] = digit_start(s_chain, (uint32_t)tid, s_back != 0);
-  __syncthreads();
+  // next tile: (chain, tile inside the chain) packed as chain << 20 | tile
+  auto take = [&]() -> uint32_t {
+    while (chains_left > 0) {
+      const uint32_t t = atomicAdd(&a.counters[my_chain], 1u);
+      const uint32_t lo = chain_lo(my_chain), hi = chain_hi(my_chain);
+      if (t < (hi - lo + TILE - 1) / TILE) return (my_chain << 20) | t;
+      my_chain += chain_step;
+      if (my_chain >= a.chains) my_chain -= a.chains;
+      --chains_left;
+    }
+    return kNoWork;
+  };
 
-  uint32_t chain = s_chain, tile = s_tile;
-  bool back = s_back != 0;
-  uint32_t clo = chain_lo(chain), chi = chain_hi(chain);
+  if (tid == taker) s_tile = take();
+  __syncthreads();
+  uint32_t cur = s_tile;
+  uint32_t cur_chain = 0xFFFFFFFFu, cur_base = 0, clo = 0, chi = 0;
   uint32_t kc[IPT];
   auto load = [&](uint32_t lo, uint32_t hi, uint32_t t, uint32_t (&k)[IPT]) {
     const uint32_t tb = lo + t * (uint32_t)TILE;
@@ -387,10 +395,15 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
       }
     }
   };
-  load(clo, chi, tile, kc);
+  if (cur != kNoWork) load(chain_lo(cur >> 20), chain_hi(cur >> 20), cur & 0xFFFFFu, kc);
 
-  for (;;) {
-    const uint32_t tb = clo + tile * (uint32_t)TILE;
+  while (cur != kNoWork) {
+    const uint32_t chain = cur >> 20, lt = cur & 0xFFFFFu;
+    if (chain != cur_chain) {
+      clo = chain_lo(chain);
+      chi = chain_hi(chain);
+    }
+    const uint32_t tb = clo + lt * (uint32_t)TILE;
     const uint32_t tile_cnt = min((uint32_t)TILE, chi - tb);
     const bool full = tile_cnt == (uint32_t)TILE;
     const uint32_t wb = tb + (uint32_t)(w * IPT * kWave + lane);
@@ -411,10 +424,20 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
 #pragma unroll
     for (int i = 0; i < IPT; ++i) asm volatile("" : "+v"(kc[i]));
     __syncthreads();
-    // next tile of the same chain and side, taken while the counts are scanned
-    if (tid == 0) s_tile = msd_take(a.ctr, chain, back);
+
     uint32_t total = 0;
-    if (tid < kRadix) {
+    D* chain_desc = a.desc + (size_t)a.tile_base[chain] * kRadix;
+    D win[kPassWindow];
+    uint32_t taken = kNoWork;
+    if (lb_wave) {
+      const uint32_t d = (uint32_t)tid - kRadix;
+      if (tid == taker) taken = take();
+#pragma unroll
+      for (int i = 0; i < W; ++i) total += off[i][d];
+      msd_publish(chain_desc + (size_t)lt * kRadix + d,
+                  Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)total, a.parity), xcd_local);
+      if (lt > 0) lookback_load(chain_desc, (int64_t)lt - 1, d, win);
+    } else if (tid < kRadix) {
 #pragma unroll
       for (int i = 0; i < W; ++i) total += off[i][tid];
     }
@@ -427,19 +450,17 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
         off[i][tid] = r;
         r += c;
       }
-      // output start of this tile's digit run, then advance the chain's
-      // running offset past (front) or below (back) it
-      uint32_t start;
-      if (!back) {
-        start = run[tid];
-        run[tid] = start + total;
-      } else {
-        start = run[tid] - total;
-        run[tid] = start;
-      }
-      gbase[tid] = start - excl;
+      if (chain != cur_chain) cur_base = digit_base(chain, (uint32_t)tid);
+      s_prefix[tid] = cur_base - excl;
     }
+    cur_chain = chain;
+    if (tid == taker) s_tile = taken;
     __syncthreads();
+    int64_t lb_t = (int64_t)lt - 1;
+    uint32_t lb_prefix = 0;
+    bool lb_done = false;
+    if (lb_wave && lt > 0)
+      lb_done = lookback_step(chain_desc, lb_t, (uint32_t)tid - kRadix, a.parity, win, lb_prefix);
     if (full) {
       uint32_t pos[IPT];
 #pragma unroll
@@ -454,8 +475,25 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
       }
     }
     const uint32_t nxt = s_tile;
-    if (nxt != kNoWork) load(clo, chi, nxt, kc);
+    if (nxt != kNoWork) {
+      const uint32_t nc = nxt >> 20;
+      load(chain_lo(nc), chain_hi(nc), nxt & 0xFFFFFu, kc);
+    }
     for (int i = lane; i < kRadix; i += kWave) off[w][i] = 0;
+    if (lb_wave) {
+      const uint32_t d = (uint32_t)tid - kRadix;
+      if (lt > 0) {
+        bool gave_up = false;
+        const uint32_t prefix =
+            lb_done ? lb_prefix : lookback_from(chain_desc, lb_t, d, a.parity, win, lb_prefix, a.spin_limit, gave_up);
+        if (gave_up) atomicOr(a.status, kStatusLookback);
+        msd_publish(chain_desc + (size_t)lt * kRadix + d, Desc<D>::make(kStInclusive, (D)(prefix + total), a.parity),
+                    xcd_local);
+        gbase[d] = s_prefix[d] + prefix;
+      } else {
+        gbase[d] = s_prefix[d];
+      }
+    }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -467,154 +505,176 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
       }
     }
     __syncthreads();
-    if (nxt != kNoWork) {
-      tile = nxt;
-      continue;
-    }
-    // this side of the chain is done: help another chain from its back end
-    if (tid == 0) {
-      uint32_t c = kNoWork, t = kNoWork;
-      msd_find_back(a.ctr, a.claim, a.chains, chain + 1, c, t);
-      s_chain = c;
-      s_tile = t;
-    }
-    __syncthreads();
-    if (s_tile == kNoWork) break;
-    chain = s_chain;
-    tile = s_tile;
-    back = true;
-    clo = chain_lo(chain);
-    chi = chain_hi(chain);
-    if (tid < kRadix) run[tid] = digit_start(chain, (uint32_t)tid, true);
-    load(clo, chi, tile, kc);
-    __syncthreads();
+    cur = nxt;
   }
 }
 
 // ---------------------------------------------------------------------------
-// local sort of the sub-bucket batches, in place
+// batch table: workgroup b, thread k < batch_cnt[b] writes batch
+// batch_base[b] + k of bucket b as {first key, key count | several
+// sub-buckets << 31}
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void msd_batches_kernel(const MsdCtl* __restrict__ ctl,
+                                                          const uint32_t* __restrict__ bucket_start,
+                                                          const uint32_t* __restrict__ sub_off,
+                                                          const uint32_t* __restrict__ leaders,
+                                                          const uint32_t* __restrict__ batch_cnt,
+                                                          const uint32_t* __restrict__ batch_base,
+                                                          uint2* __restrict__ batches) {
+  if (ctl->off) return;
+  const uint32_t b = blockIdx.x, k = threadIdx.x;
+  const uint32_t cnt = batch_cnt[b];
+  if (k >= cnt) return;
+  const uint32_t s0 = leaders[b * kRadix + k];
+  const uint32_t s1 = k + 1 < cnt ? leaders[b * kRadix + k + 1] : (uint32_t)kRadix;
+  const uint32_t beg = bucket_start[b] + sub_off[b * kRadix + s0];
+  const uint32_t end = s1 < (uint32_t)kRadix ? bucket_start[b] + sub_off[b * kRadix + s1] : bucket_start[b + 1];
+  batches[batch_base[b] + k] = make_uint2(beg, (end - beg) | (s1 - s0 > 1 ? 0x80000000u : 0u));
+}
+
+// ---------------------------------------------------------------------------
+// local sort of the batches, in place: workgroup g sorts batches g, g + G,
+// g + 2G, ... (G workgroups).  A batch of one sub-bucket is sorted by bytes
+// 0 and 1 (its top 16 bits are one value), a batch of several by bytes 0, 1
+// and 2 (one bucket: byte 3 is one value).  The next batch's keys are loaded
+// into registers while the current one is sorted in LDS.  Block 0 also
+// checks that both passes took every tile of every chain.
 // ---------------------------------------------------------------------------
 template <int KT>
 __global__ __launch_bounds__(kMsdBlock, 4) void msd_local_kernel(uint32_t* __restrict__ data,
-                                                                 MsdCtl* __restrict__ ctl,
+                                                                 const MsdCtl* __restrict__ ctl,
+                                                                 const uint2* __restrict__ batches,
+                                                                 const uint32_t* __restrict__ counters,
                                                                  const uint32_t* __restrict__ bucket_start,
-                                                                 const uint32_t* __restrict__ sub_off,
-                                                                 const uint32_t* __restrict__ leaders,
-                                                                 const uint32_t* __restrict__ batch_cnt,
-                                                                 const uint32_t* __restrict__ batch_base) {
+                                                                 uint32_t chains1, uint32_t chain_keys1, uint32_t n,
+                                                                 uint32_t* __restrict__ status) {
   constexpr int W = kMsdBlock / kWave;
   constexpr int IPT = kLocalIpt;
   if (ctl->off) return;
-  __shared__ uint32_t keys[kLocalCap];
-  __shared__ uint32_t off[W][kRadix];
+  // keys[kLocalCap + lane]: where the lanes without a key (beyond the
+  // batch) store, so that every LDS access is unconditional
+  __shared__ uint32_t keys[kLocalCap + kWave];
+  // off[w][kRadix + lane]: the counters lanes without a key count into (one
+  // per lane: a shared one would serialise the atomics)
+  __shared__ uint32_t off[W][kRadix + kWave];
   __shared__ uint32_t sums[4];
-  __shared__ uint32_t s_batch;
-  __shared__ uint32_t s_base[kRadix + 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int w = tid / kWave;
-  const uint32_t total = ctl->batches;
-  for (int i = tid; i <= kRadix; i += kMsdBlock) s_base[i] = batch_base[i];
-  for (int i = tid; i < W * kRadix; i += kMsdBlock) (&off[0][0])[i] = 0;
-  const uint32_t wb = (uint32_t)(w * IPT * kWave + lane);  // wave-blocked item positions
-
-  // bounds of batch bi: [beg, end) and whether it holds several sub-buckets
-  auto bounds = [&](uint32_t bi, uint32_t& beg, uint32_t& end, bool& multi) {
-    uint32_t lo = 0, hi = kRadix;  // largest b with s_base[b] <= bi
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (s_base[mid] <= bi) lo = mid;
-      else hi = mid;
-    }
-    const uint32_t b = lo, k = bi - s_base[b];
-    const uint32_t s0 = leaders[b * kRadix + k];
-    const uint32_t s1 = k + 1 < batch_cnt[b] ? leaders[b * kRadix + k + 1] : (uint32_t)kRadix;
-    beg = bucket_start[b] + sub_off[b * kRadix + s0];
-    end = s1 < (uint32_t)kRadix ? bucket_start[b] + sub_off[b * kRadix + s1] : bucket_start[b + 1];
-    multi = s1 - s0 > 1;
-  };
-
-  if (tid == 0) s_batch = atomicAdd(&ctl->batch_next, 1u);
-  __syncthreads();
-  uint32_t bi = s_batch;
-  uint32_t beg = 0, end = 0;
-  bool multi = false;
-  uint32_t kc[IPT];
-  if (bi < total) {
-    bounds(bi, beg, end, multi);
-#pragma unroll
-    for (int i = 0; i < IPT; ++i) {
-      const uint32_t p = wb + i * kWave;
-      kc[i] = p < end - beg ? data[beg + p] : 0u;
+  if (blockIdx.x == 0) {
+    // counters: [0, 256) pass 2 per bucket, [256, 264) pass 1 per chain
+    if (tid < kRadix) {
+      const uint32_t len = bucket_start[tid + 1] - bucket_start[tid];
+      if (counters[tid] < (len + kMsdTile - 1) / kMsdTile) atomicOr(status, kStatusChainUnserved);
+    } else if (tid < kRadix + (int)chains1) {
+      const uint32_t c = tid - kRadix;
+      const uint32_t lo = c * chain_keys1;
+      const uint32_t len = min(n - lo, chain_keys1);
+      if (counters[tid] < (len + kMsdTile - 1) / kMsdTile) atomicOr(status, kStatusChainUnserved);
     }
   }
-  while (bi < total) {
-    const uint32_t m = end - beg;
-    const int passes = multi ? 3 : 2;
+  const uint32_t total = ctl->batches;
+  const uint32_t G = gridDim.x;
+  for (int i = tid; i < W * (kRadix + kWave); i += kMsdBlock) (&off[0][0])[i] = 0;
+  // item i of a lane sits at wb + 64 i: wavefront w holds the positions
+  // [w IPT 64, (w+1) IPT 64), in the order the ranking keeps stable (a
+  // batch smaller than kLocalCap leaves the last wavefronts idle: measured
+  // faster than spreading it over all of them, which gives each wavefront
+  // fewer LDS atomics in flight)
+  const uint32_t wb = (uint32_t)(w * IPT * kWave + lane);
+  // (unconditional loads: positions beyond the batch read its last key, or
+  // key 0 for an empty batch, and are ignored)
+  auto load_batch = [&](uint2 bt, uint32_t (&k)[IPT]) {
+    const uint32_t m = bt.y & 0x7FFFFFFFu;
+    const uint32_t* src = data + (m ? bt.x : 0u);
+    const uint32_t last = m ? m - 1 : 0u;
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) k[i] = src[min(wb + i * kWave, last)];
+  };
+
+  uint32_t bi = blockIdx.x;
+  uint2 cur = bi < total ? batches[bi] : make_uint2(0, 0);
+  uint32_t kc[IPT];
+  load_batch(cur, kc);
+
+  // one LSD pass over byte shift/8 of the m keys in k into LDS, in order;
+  // branch-free, so that each wavefront's LDS operations issue back to back
+  auto lds_pass = [&](const uint32_t (&k)[IPT], int shift, uint32_t m) {
+    // a wavefront whose positions all lie beyond the batch skips its LDS
+    // work (a wave-uniform branch)
+    const bool wave_has_keys = (uint32_t)(w * IPT * kWave) < m;
+    uint32_t dg[IPT], pk[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+      dg[i] = wb + i * kWave < m ? digit_of<KT>(k[i], shift) : (uint32_t)(kRadix + lane);
+    if (wave_has_keys) rank_items_full<IPT, false>(dg, off[w], pk);
     __syncthreads();
-    if (tid == 0) s_batch = atomicAdd(&ctl->batch_next, 1u);
-    for (int q = 0; q < passes; ++q) {
-      const int shift = 8 * q;
+    uint32_t tot = 0;
+    if (tid < kRadix) {
+#pragma unroll
+      for (int i = 0; i < W; ++i) tot += off[i][tid];
+    }
+    const uint32_t excl = block_excl_scan_256(tid < kRadix ? tot : 0u, sums, tid);
+    if (tid < kRadix) {
+      uint32_t r = excl;
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const uint32_t c = off[i][tid];
+        off[i][tid] = r;
+        r += c;
+      }
+    }
+    __syncthreads();
+    if (wave_has_keys) {
+      uint32_t pos[IPT];
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) pos[i] = off[w][pk[i] >> 16];  // lanes without a key read their spare counter
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+        pos[i] = (pk[i] >> 16) < (uint32_t)kRadix ? min(pos[i] + (pk[i] & 0xFFFFu), kLocalCap - 1)
+                                                  : kLocalCap + (uint32_t)lane;
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) keys[pos[i]] = k[i];
+    }
+    // every row is cleared: the scan wrote running offsets into the rows of
+    // the wavefronts without keys as well
+    for (int i = lane; i < kRadix + kWave; i += kWave) off[w][i] = 0;
+  };
+
+  while (bi < total) {
+    const uint32_t beg = cur.x, m = cur.y & 0x7FFFFFFFu;
+    const int passes = (cur.y >> 31) ? 3 : 2;
+    const uint32_t nb = bi + G;
+    const uint2 nxt = nb < total ? batches[nb] : make_uint2(0, 0);
+    __syncthreads();
+    lds_pass(kc, 0, m);
+    // the batch is in LDS: load the next one into registers meanwhile
+    if (nb < total) load_batch(nxt, kc);
+    __syncthreads();
+    for (int q = 1; q < passes; ++q) {
       uint32_t k[IPT];
-      if (q == 0) {
 #pragma unroll
-        for (int i = 0; i < IPT; ++i) k[i] = kc[i];
-      } else {
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-          const uint32_t p = wb + i * kWave;
-          k[i] = p < m ? keys[p] : 0u;
-        }
-      }
-      uint32_t pk[IPT];
-#pragma unroll
-      for (int i = 0; i < IPT; ++i) {
-        const uint32_t d = digit_of<KT>(k[i], shift);
-        pk[i] = (wb + i * kWave < m) ? ((d << 16) | rank_wave_atomic<false>(d, off[w])) : 0u;
-      }
-      __syncthreads();
-      uint32_t tot = 0;
-      if (tid < kRadix) {
-#pragma unroll
-        for (int i = 0; i < W; ++i) tot += off[i][tid];
-      }
-      const uint32_t excl = block_excl_scan_256(tid < kRadix ? tot : 0u, sums, tid);
-      if (tid < kRadix) {
-        uint32_t r = excl;
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-          const uint32_t c = off[i][tid];
-          off[i][tid] = r;
-          r += c;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < IPT; ++i) {
-        const uint32_t pos = off[w][pk[i] >> 16] + (pk[i] & 0xFFFFu);
-        if (wb + i * kWave < m && pos < kLocalCap) keys[pos] = k[i];
-      }
-      for (int i = lane; i < kRadix; i += kWave) off[w][i] = 0;
-      if (q == 0) {
-        // the batch is in LDS: load the next one into registers meanwhile
-        const uint32_t nb = s_batch;
-        if (nb < total) {
-          uint32_t b2, e2;
-          bool m2;
-          bounds(nb, b2, e2, m2);
-#pragma unroll
-          for (int i = 0; i < IPT; ++i) {
-            const uint32_t p = wb + i * kWave;
-            kc[i] = p < e2 - b2 ? data[b2 + p] : 0u;
-          }
-        }
-      }
+      for (int i = 0; i < IPT; ++i) k[i] = keys[min(wb + i * kWave, kLocalCap + kWave - 1)];
+      lds_pass(k, 8 * q, m);
       __syncthreads();
     }
-    for (uint32_t i = (uint32_t)tid; i < m; i += kMsdBlock) data[beg + i] = keys[i];
-    bi = s_batch;
-    if (bi < total) bounds(bi, beg, end, multi);
+    // write-out: all LDS reads first, then the stores
+    {
+      uint32_t v[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t i = (uint32_t)(j * kMsdBlock + tid);
+        v[j] = i < m ? keys[i] : 0u;
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t i = (uint32_t)(j * kMsdBlock + tid);
+        if (i < m) data[beg + i] = v[j];
+      }
+    }
+    bi = nb;
+    cur = nxt;
   }
 }
 
@@ -659,9 +719,12 @@ bool msd_eligible(size_t n) {
          env_u32("MGS_MSD", 1) != 0;
 }
 
-// Scratch of the path (at most MsdCfg::kMaxChains chains), placed from
-// offset `at` of the temp buffer.
-MsdLayout msd_layout(size_t at) {
+// Scratch of the path (at most MsdCfg::kMaxChains upfront ranges) from
+// offset `at` of temp: the look-back descriptors of its passes first (they
+// may overlap the LSD sort's, which only a sort the path turned itself off
+// for uses), then its tables; the control block goes at `ctl_at`, which must
+// lie outside what the LSD sort clears or writes.
+MsdLayout msd_layout(size_t n, size_t at, size_t ctl_at) {
   MsdLayout L{};
   size_t o = at;
   auto take = [&](size_t bytes) {
@@ -670,11 +733,14 @@ MsdLayout msd_layout(size_t at) {
     return r;
   };
   constexpr size_t C = MsdCfg::kMaxChains;
-  L.ctl = take(sizeof(MsdCtl));
+  const size_t tiles = (n + kMsdTile - 1) / kMsdTile;
+  L.ctl = ctl_at;
+  L.desc = take((tiles + kRadix + kXcds) * kRadix * 4);
   L.slices = take(C * (kMsdBins / 2) * 4);
   L.chain_hist = take(C * kRadix * 4);
   L.chain_pref = take(C * kRadix * 4);
   L.chain_bad = take(C * 4);
+  L.xcd_of = take(C * 4);
   L.sub_cnt = take((size_t)kMsdBins * 4);
   L.sub_off = take((size_t)kMsdBins * 4);
   L.bucket_tot = take(kRadix * 4);
@@ -682,93 +748,108 @@ MsdLayout msd_layout(size_t at) {
   L.leaders = take((size_t)kMsdBins * 4);
   L.batch_cnt = take(kRadix * 4);
   L.batch_base = take((kRadix + 1) * 4);
-  L.ctr1 = take(C * 8);
-  L.ctr2 = take(kRadix * 8);
-  L.claim1 = take(C * 4);
-  L.claim2 = take(kRadix * 4);
+  L.batches = take((size_t)kMsdBins * 8);
+  L.tile_base1 = take(kXcds * 4);
+  L.tile_base2 = take(kRadix * 4);
+  L.counters = take((kRadix + kXcds) * 4);
   L.end = o;
   return L;
 }
 
 template <int KT>
 int msd_launch(const void* kin, void* kout, void* alt, size_t n, char* t, const MsdLayout& L, uint32_t* status,
-               hipStream_t s, Marks& mark) {
+               uint32_t spin_limit, hipStream_t s, Marks& mark) {
   const uint32_t* ki = static_cast<const uint32_t*>(kin);
   uint32_t* ko = static_cast<uint32_t*>(kout);
   uint32_t* ka = static_cast<uint32_t*>(alt);
+  auto at = [&](size_t o) { return reinterpret_cast<uint32_t*>(t + o); };
   MsdCtl* ctl = reinterpret_cast<MsdCtl*>(t + L.ctl);
-  uint32_t* slices = reinterpret_cast<uint32_t*>(t + L.slices);
-  uint32_t* chain_hist = reinterpret_cast<uint32_t*>(t + L.chain_hist);
-  uint32_t* chain_pref = reinterpret_cast<uint32_t*>(t + L.chain_pref);
-  uint32_t* chain_bad = reinterpret_cast<uint32_t*>(t + L.chain_bad);
-  uint32_t* sub_cnt = reinterpret_cast<uint32_t*>(t + L.sub_cnt);
-  uint32_t* sub_off = reinterpret_cast<uint32_t*>(t + L.sub_off);
-  uint32_t* bucket_tot = reinterpret_cast<uint32_t*>(t + L.bucket_tot);
-  uint32_t* bucket_start = reinterpret_cast<uint32_t*>(t + L.bucket_start);
-  uint32_t* leaders = reinterpret_cast<uint32_t*>(t + L.leaders);
-  uint32_t* batch_cnt = reinterpret_cast<uint32_t*>(t + L.batch_cnt);
-  uint32_t* batch_base = reinterpret_cast<uint32_t*>(t + L.batch_base);
-  uint64_t* ctr1 = reinterpret_cast<uint64_t*>(t + L.ctr1);
-  uint64_t* ctr2 = reinterpret_cast<uint64_t*>(t + L.ctr2);
-  uint32_t* claim1 = reinterpret_cast<uint32_t*>(t + L.claim1);
-  uint32_t* claim2 = reinterpret_cast<uint32_t*>(t + L.claim2);
+  uint32_t* desc = at(L.desc);
+  uint32_t* slices = at(L.slices);
+  uint32_t* chain_hist = at(L.chain_hist);
+  uint32_t* chain_pref = at(L.chain_pref);
+  uint32_t* chain_bad = at(L.chain_bad);
+  uint32_t* xcd_of = at(L.xcd_of);
+  uint32_t* sub_cnt = at(L.sub_cnt);
+  uint32_t* sub_off = at(L.sub_off);
+  uint32_t* bucket_tot = at(L.bucket_tot);
+  uint32_t* bucket_start = at(L.bucket_start);
+  uint32_t* leaders = at(L.leaders);
+  uint32_t* batch_cnt = at(L.batch_cnt);
+  uint32_t* batch_base = at(L.batch_base);
+  uint2* batches = reinterpret_cast<uint2*>(t + L.batches);
+  uint32_t* tile_base1 = at(L.tile_base1);
+  uint32_t* tile_base2 = at(L.tile_base2);
+  uint32_t* counters = at(L.counters);
   const uint32_t nn = (uint32_t)n;
 
-  // one chain (and one workgroup of each pass) per resident pass workgroup
-  uint32_t chains = msd_resident(msd_pass_kernel<KT, 1>);
-  chains = std::max<uint32_t>(1, std::min<uint32_t>(chains ? chains : 256u, MsdCfg::kMaxChains));
-  const size_t tiles = (n + kMsdTile - 1) / kMsdTile;
-  const uint32_t chain_keys = (uint32_t)((tiles + chains - 1) / chains * kMsdTile);
-  chains = (uint32_t)((n + chain_keys - 1) / chain_keys);
+  // upfront ranges: up to kMaxChains, a whole number of tiles each, grouped
+  // 32 to a pass-1 chain (8 chains at full size)
+  const uint32_t tiles = (uint32_t)((n + kMsdTile - 1) / kMsdTile);
+  const uint32_t tpr = (tiles + MsdCfg::kMaxChains - 1) / MsdCfg::kMaxChains;
+  const uint32_t range_keys = tpr * kMsdTile;
+  const uint32_t ranges = (uint32_t)((n + range_keys - 1) / range_keys);
+  constexpr uint32_t kRangesPerChain = MsdCfg::kMaxChains / kXcds;
+  const uint32_t chains1 = (ranges + kRangesPerChain - 1) / kRangesPerChain;
+  // workgroups of the passes and the local sort: as many as are resident
+  uint32_t grid = msd_resident(msd_pass_kernel<KT, 1>);
+  grid = grid ? grid : 256u;
 
   mark(s, "msd_upfront");
   if ((reinterpret_cast<uintptr_t>(ki) % 16) == 0)
-    msd_upfront_kernel<KT, true><<<chains, kMsdBlock, 0, s>>>(ki, nn, chain_keys, ctl, slices, chain_hist, chain_bad,
-                                                              status);
+    msd_upfront_kernel<KT, true><<<ranges, kMsdBlock, 0, s>>>(ki, nn, range_keys, ctl, slices, chain_hist, chain_bad,
+                                                              xcd_of, desc, tiles, status);
   else
-    msd_upfront_kernel<KT, false><<<chains, kMsdBlock, 0, s>>>(ki, nn, chain_keys, ctl, slices, chain_hist, chain_bad,
-                                                               status);
+    msd_upfront_kernel<KT, false><<<ranges, kMsdBlock, 0, s>>>(ki, nn, range_keys, ctl, slices, chain_hist, chain_bad,
+                                                               xcd_of, desc, tiles, status);
   mark(s, "msd_scan");
-  msd_scan_kernel<<<kRadix, kRadix, 0, s>>>(slices, chains, nn, chain_hist, chain_pref, ctl, sub_cnt, sub_off,
-                                            bucket_tot, leaders, batch_cnt, chain_bad);
+  msd_scan_kernel<<<kRadix, kRadix, 0, s>>>(slices, ranges, nn, chain_hist, chain_pref, ctl, sub_cnt, sub_off,
+                                            bucket_tot, leaders, batch_cnt, chain_bad, xcd_of);
   mark(s, "msd_finish");
-  msd_finish_kernel<<<1, kRadix, 0, s>>>(ctl, bucket_tot, bucket_start, batch_cnt, batch_base, nn, chain_keys, chains,
-                                         ctr1, ctr2, claim1, claim2);
+  msd_finish_kernel<<<1, kRadix, 0, s>>>(ctl, bucket_tot, bucket_start, batch_cnt, batch_base, chains1,
+                                         kRangesPerChain * tpr, tile_base1, tile_base2, counters);
   MsdPassArgs a{};
   a.n = nn;
-  a.chain_keys = chain_keys;
   a.bucket_start = bucket_start;
-  a.chain_hist = chain_hist;
   a.chain_pref = chain_pref;
-  a.sub_cnt = sub_cnt;
+  a.ranges_per_chain = kRangesPerChain;
+  a.chain_keys = kRangesPerChain * range_keys;
   a.sub_off = sub_off;
+  a.desc = desc;
   a.ctl = ctl;
+  a.status = status;
+  a.spin_limit = spin_limit;
   mark(s, "msd_pass1");
   a.src = ki;
   a.dst = ka;
-  a.chains = chains;
+  a.chains = chains1;
+  a.tile_base = tile_base1;
+  a.counters = counters + kRadix;
   a.shift = 24;
-  a.ctr = ctr1;
-  a.claim = claim1;
-  msd_pass_kernel<KT, 1><<<chains, kMsdBlock, 0, s>>>(a);
+  a.parity = 0;
+  msd_pass_kernel<KT, 1><<<std::min(grid, tiles), kMsdBlock, 0, s>>>(a);
   mark(s, "msd_pass2");
   a.src = ka;
   a.dst = ko;
   a.chains = kRadix;
+  a.tile_base = tile_base2;
+  a.counters = counters;
   a.shift = 16;
-  a.ctr = ctr2;
-  a.claim = claim2;
-  msd_pass_kernel<KT, 2><<<chains, kMsdBlock, 0, s>>>(a);
+  a.parity = 1;
+  msd_pass_kernel<KT, 2><<<grid, kMsdBlock, 0, s>>>(a);
+  mark(s, "msd_batches");
+  msd_batches_kernel<<<kRadix, kRadix, 0, s>>>(ctl, bucket_start, sub_off, leaders, batch_cnt, batch_base, batches);
   mark(s, "msd_local");
-  msd_local_kernel<KT><<<chains, kMsdBlock, 0, s>>>(ko, ctl, bucket_start, sub_off, leaders, batch_cnt, batch_base);
+  msd_local_kernel<KT><<<grid, kMsdBlock, 0, s>>>(ko, ctl, batches, counters, bucket_start, chains1,
+                                                  kRangesPerChain * range_keys, nn, status);
   return check_launch();
 }
 
-template int msd_launch<MGS_KEY_U32>(const void*, void*, void*, size_t, char*, const MsdLayout&, uint32_t*,
+template int msd_launch<MGS_KEY_U32>(const void*, void*, void*, size_t, char*, const MsdLayout&, uint32_t*, uint32_t,
                                      hipStream_t, Marks&);
-template int msd_launch<MGS_KEY_I32>(const void*, void*, void*, size_t, char*, const MsdLayout&, uint32_t*,
+template int msd_launch<MGS_KEY_I32>(const void*, void*, void*, size_t, char*, const MsdLayout&, uint32_t*, uint32_t,
                                      hipStream_t, Marks&);
-template int msd_launch<MGS_KEY_F32>(const void*, void*, void*, size_t, char*, const MsdLayout&, uint32_t*,
+template int msd_launch<MGS_KEY_F32>(const void*, void*, void*, size_t, char*, const MsdLayout&, uint32_t*, uint32_t,
                                      hipStream_t, Marks&);
 
 }  // namespace mgs

@@ -17,15 +17,16 @@ struct MsdCfg {
   static constexpr int kBlock = 1024;  // one workgroup per CU
   static constexpr int kIpt = 18;      // pass tiles of 18432 keys
   static constexpr int kTile = kBlock * kIpt;
-  // local sort: up to 24 keys per thread, 24576 keys (96 KB) in LDS
-  static constexpr int kLocalIpt = 24;
+  // local sort: up to 20 keys per thread, 20480 keys (80 KB) in LDS (the
+  // largest of the 65536 sub-buckets of 2^30 uniform keys holds about 17000)
+  static constexpr int kLocalIpt = 20;
   static constexpr uint32_t kLocalCap = (uint32_t)kBlock * kLocalIpt;
   static constexpr uint32_t kMaxChains = 256;
   // sizes the path is tried for (MGS_MSD_MIN overrides the lower bound):
   // from 2^26 keys, and up to where the mean sub-bucket (n / 65536) still
   // leaves room below kLocalCap
   static constexpr size_t kMinKeys = size_t(1) << 26;
-  static constexpr size_t kMaxKeys = (size_t(1) << 30) + (size_t(1) << 28);
+  static constexpr size_t kMaxKeys = (size_t(1) << 30) + (size_t(1) << 27);
 };
 
 // Control block.  off: any set bit turns the path off (a 16-bit counter
@@ -35,22 +36,25 @@ struct MsdCtl {
   uint32_t off;
   uint32_t batches;
   uint32_t batch_next;
-  uint32_t pad;
+  uint32_t xcd_mask;  // XCDs the upfront kernel ran on
 };
 
 struct MsdLayout {
-  size_t ctl, slices, chain_hist, chain_pref, chain_bad, sub_cnt, sub_off, bucket_tot, bucket_start, leaders,
-      batch_cnt, batch_base, ctr1, ctr2, claim1, claim2, end;
+  size_t ctl, desc, slices, chain_hist, chain_pref, chain_bad, xcd_of, sub_cnt, sub_off, bucket_tot, bucket_start,
+      leaders, batch_cnt, batch_base, batches, tile_base1, tile_base2, counters, end;
 };
 
 // n in the range the path is tried for (and MGS_MSD not set to 0).
 bool msd_eligible(size_t n);
-MsdLayout msd_layout(size_t at);
+// The path's scratch from offset `at` of temp (it may overlap the LSD
+// sort's descriptors, which only a sort the path turned itself off for
+// uses), its control block at `ctl_at` (which must not).
+MsdLayout msd_layout(size_t n, size_t at, size_t ctl_at);
 
 // Enqueues the path's kernels: keys from kin, pass 1 into alt, pass 2 and
 // the local sort into kout.  t is the temp buffer the layout indexes.
 template <int KT>
 int msd_launch(const void* kin, void* kout, void* alt, size_t n, char* t, const MsdLayout& L, uint32_t* status,
-               hipStream_t s, Marks& mark);
+               uint32_t spin_limit, hipStream_t s, Marks& mark);
 
 }  // namespace mgs

@@ -704,7 +704,7 @@ bool rank_peers_requested() {
 }
 
 struct SortLayout {
-  size_t status, alt_k, alt_v, hist, red, counters, desc, total;
+  size_t status, alt_k, alt_v, hist, red, counters, msd_ctl, desc, total;
   uint32_t num_tiles;
   bool wide_desc;
 };
@@ -729,6 +729,7 @@ SortLayout sort_layout_cfg(size_t n) {
   L.hist = take((size_t)KB * kChains * kRadix * 4);
   L.red = take(3 * sizeof(uint64_t));
   L.counters = take((size_t)KB * kChains * 4);
+  L.msd_ctl = take(sizeof(MsdCtl));  // the three-pass path's decision word: read by the LSD kernels
   L.desc = take((size_t)L.num_tiles * kRadix * (L.wide_desc ? 8 : 4));
   L.total = o;
   return L;
@@ -747,7 +748,7 @@ template <int KB, int VB>
 size_t sort_bytes(size_t n) {
   const SortLayout L = sort_layout<KB, VB>(n);
   size_t b = L.total;
-  if (KB == 4 && VB == 0 && msd_eligible(n)) b = std::max(b, msd_layout(L.desc).end);
+  if (KB == 4 && VB == 0 && msd_eligible(n)) b = std::max(b, msd_layout(n, L.desc, L.msd_ctl).end);
   if (KB + VB > 4 && n >= kBigTileMin) b = std::max(b, sort_layout_cfg<KB, VB, false>(kBigTileMin - 1).total);
   return b;
 }
@@ -804,7 +805,7 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
   // 4-byte keys alone: the three-pass path first (its scratch overlaps the
   // LSD descriptors, which only a sort it turned itself off for uses)
   const bool msd = KB == 4 && VB == 0 && msd_eligible(n);
-  const MsdLayout ML = msd_layout(L.desc);
+  const MsdLayout ML = msd_layout(n, L.desc, L.msd_ctl);
   const size_t need = msd ? std::max(L.total, ML.end) : L.total;
   if (temp == nullptr || temp_bytes < need) return MGS_ERR_TEMP_TOO_SMALL;
   char* t = static_cast<char*>(temp);
@@ -822,7 +823,7 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
   const uint32_t* gate = nullptr;
   if constexpr (KB == 4 && VB == 0) {
     if (msd) {
-      const int rc = msd_launch<KT>(kin, kout, ka, n, t, ML, status, s, mark);
+      const int rc = msd_launch<KT>(kin, kout, ka, n, t, ML, status, spin_limit, s, mark);
       if (rc != MGS_OK) return rc;
       gate = reinterpret_cast<const uint32_t*>(t + ML.ctl);  // MsdCtl::off
     }
@@ -946,7 +947,7 @@ int path_of(size_t n, const void* temp, size_t temp_bytes, hipStream_t s) {
   if (n <= (size_t)SCfg::kTile) return MGS_PATH_SMALL;
   if (!(KB == 4 && VB == 0 && msd_eligible(n))) return MGS_PATH_LSD;
   const SortLayout L = sort_layout<KB, VB>(n);
-  const MsdLayout ML = msd_layout(L.desc);
+  const MsdLayout ML = msd_layout(n, L.desc, L.msd_ctl);
   if (temp == nullptr || temp_bytes < ML.end) return MGS_ERR_TEMP_TOO_SMALL;
   uint32_t off = 0;
   hipError_t e = hipMemcpyAsync(&off, static_cast<const char*>(temp) + ML.ctl, sizeof(off), hipMemcpyDeviceToHost, s);

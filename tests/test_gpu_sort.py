@@ -254,16 +254,30 @@ def test_lookback_bound_reports_status(mg, kt, vdt, monkeypatch):
 MSD_KEYS = [O.K_U32, O.K_I32, O.K_F32]
 
 
+def expected_path(kt, keys):
+    """The three-pass path hands over to the LSD passes when a 16-bit
+    sub-bucket exceeds the 20480 keys of the LDS sort or a byte-3 bucket
+    holds more than twice its share plus two 18432-key tiles."""
+    n = keys.size
+    if n <= 4096:
+        return 1
+    top = (O.encode(kt, keys) >> np.uint64(16)).astype(np.int64)
+    sub = np.bincount(top, minlength=1 << 16)
+    bucket = sub.reshape(256, 256).sum(axis=1)
+    off = sub.max() > 20480 or bucket.max() > 2 * (n // 256) + 2 * 18432
+    return 2 if off else 3
+
+
 @pytest.mark.parametrize("kt", MSD_KEYS, ids=lambda k: KEY_NAME[k])
 @pytest.mark.parametrize("n", [4097, 100_003, 1 << 20, (1 << 21) + 777])
-@pytest.mark.parametrize("dist,path", [("uniform", 3), ("sorted", 3), ("reverse", 3), ("few", 2), ("equal", 2),
-                                       ("lowbits", 2)])
-def test_three_pass_path(mg, kt, n, dist, path, monkeypatch):
+@pytest.mark.parametrize("dist", ["uniform", "sorted", "reverse", "few", "equal", "lowbits"])
+def test_three_pass_path(mg, kt, n, dist, monkeypatch):
     """4-byte keys alone try the three-pass path (MSD pass on byte 3, pass on
     byte 2 inside each bucket, LDS sort of each batch of 16-bit sub-buckets);
     MGS_MSD_MIN=0 tries it at any size.  Keys whose top 16 bits crowd into
-    a few sub-buckets (few, equal, lowbits) hand over to the LSD passes.
-    Either way the output is the oracle's, in place and out of place."""
+    a few sub-buckets (few, equal, lowbits at the larger sizes) hand over to
+    the LSD passes.  Either way the output is the oracle's, in place and out
+    of place."""
     import torch
     monkeypatch.setenv("MGS_MSD_MIN", "0")
     rng = np.random.default_rng(n + 3 * kt)
@@ -277,8 +291,7 @@ def test_three_pass_path(mg, kt, n, dist, path, monkeypatch):
         mg.sort(KEY_NAME[kt], dk, keys_out=out, temp=temp)
         assert mg.temp_status(temp) == 0
         got = mg.lib.mgs_sort_path(kt, mg.VAL_NONE, n, mg._ptr(temp), temp.numel(), None)
-        small = n <= 4096
-        assert got == (mg.PATH_SMALL if small else path), (inplace, got)
+        assert got == expected_path(kt, keys), (inplace, got)
         assert np.array_equal(from_dev(out, keys.dtype), ek), inplace
         if not inplace:
             assert np.array_equal(from_dev(dk, keys.dtype), keys)
@@ -298,5 +311,5 @@ def test_three_pass_default_size(mg, dist):
     mg.sort("u32", dk, temp=temp)
     assert mg.temp_status(temp) == 0
     path = mg.lib.mgs_sort_path(kt, mg.VAL_NONE, n, mg._ptr(temp), temp.numel(), None)
-    assert path == (mg.PATH_LSD if dist == "skewed" else mg.PATH_THREE_PASS)
+    assert path == expected_path(kt, keys)
     assert np.array_equal(from_dev(dk, np.uint32), ek)
