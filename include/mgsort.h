@@ -109,6 +109,21 @@ int mgs_sort(mgs_key kt, mgs_val vt,
              const void* vals_in, void* vals_out,
              size_t n, void* temp, size_t temp_bytes, void* stream);
 
+/* mgs_sort without a temp buffer (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops).  Same kernels as mgs_sort; the scratch is a 64 MiB
+ * array in the library's device memory, one per GPU, held until the process
+ * ends, so n is limited to mgs_sort_temp_bytes(kt, vt, n) <= 64 MiB (about
+ * 15.9M u32 keys, 7.9M u64 keys or u32 pairs, 5.2M u64 keys with u32
+ * payloads); larger n returns MGS_ERR_TOO_LARGE.  Allocates nothing.  All
+ * calls on one GPU share its array: calls that overlap on one GPU (on
+ * different streams or host threads) race and must be ordered by the
+ * caller. */
+int mgs_sort_noalloc(mgs_key kt, mgs_val vt,
+                     const void* keys_in, void* keys_out,
+                     const void* vals_in, void* vals_out,
+                     size_t n, void* stream);
+
 /* Measurement variant of mgs_sort (no reference counterpart; used by the
  * benchmark to time each kernel on the stream it runs on): enqueues the same
  * kernels and records events[i] (hipEvent_t handles created by the caller)

@@ -41,6 +41,16 @@ int mgs_sort(mgs_key kt, mgs_val vt, const void* keys_in, void* keys_out, const 
   return mgs::sort_dispatch(kt, vt, keys_in, keys_out, vals_in, vals_out, n, temp, temp_bytes, as_stream(stream));
 }
 
+int mgs_sort_noalloc(mgs_key kt, mgs_val vt, const void* keys_in, void* keys_out, const void* vals_in, void* vals_out,
+                     size_t n, void* stream) {
+  if (key_bytes(kt) < 0 || val_bytes(vt) < 0) return MGS_ERR_BAD_TYPE;
+  if (n == 0) return MGS_OK;
+  if (n > kMaxKeys) return MGS_ERR_TOO_LARGE;
+  if (keys_in == nullptr || keys_out == nullptr) return MGS_ERR_INVALID;
+  if (vt != MGS_VAL_NONE && (vals_in == nullptr || vals_out == nullptr)) return MGS_ERR_INVALID;
+  return mgs::sort_noalloc(kt, vt, keys_in, keys_out, vals_in, vals_out, n, as_stream(stream));
+}
+
 int mgs_sort_timed(mgs_key kt, mgs_val vt, const void* keys_in, void* keys_out, const void* vals_in, void* vals_out,
                    size_t n, void* temp, size_t temp_bytes, void* stream, void* const* events, int num_events,
                    int* num_marks) {

@@ -57,6 +57,10 @@ int sort_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const voi
                   void* temp, size_t temp_bytes, hipStream_t s, hipEvent_t* events = nullptr, int num_events = 0,
                   int* num_marks = nullptr);
 
+// mgs_sort_noalloc (noalloc.hip).
+int sort_noalloc(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout, size_t n,
+                 hipStream_t s);
+
 // mgs_sort_path.
 int sort_path(mgs_key kt, mgs_val vt, size_t n, const void* temp, size_t temp_bytes, hipStream_t s);
 

@@ -29,7 +29,7 @@ MGS_OK = 0
 PATH_SMALL, PATH_LSD, PATH_THREE_PASS = 1, 2, 3
 
 SYMBOLS = [
-    "mgs_sort_temp_bytes", "mgs_sort", "mgs_sort_timed", "mgs_sort_timed_stages", "mgs_sort_path", "mgs_merge_temp_bytes", "mgs_merge", "mgs_select_pivot",
+    "mgs_sort_temp_bytes", "mgs_sort", "mgs_sort_noalloc", "mgs_sort_timed", "mgs_sort_timed_stages", "mgs_sort_path", "mgs_merge_temp_bytes", "mgs_merge", "mgs_select_pivot",
     "mgs_swap_ranges", "mgs_fill_max", "mgs_segmented_sort_temp_bytes", "mgs_segmented_sort_temp_bytes_tuned",
     "mgs_segmented_sort", "mgs_segmented_sort_ranges", "mgs_temp_status", "mgs_status_str", "mgs_last_hip_error",
     "mgs_version",
@@ -51,6 +51,8 @@ def _load() -> ctypes.CDLL:
     lib.mgs_sort_temp_bytes.restype = sz
     lib.mgs_sort.argtypes = [i, i, vp, vp, vp, vp, sz, vp, sz, vp]
     lib.mgs_sort.restype = i
+    lib.mgs_sort_noalloc.argtypes = [i, i, vp, vp, vp, vp, sz, vp]
+    lib.mgs_sort_noalloc.restype = i
     lib.mgs_sort_timed.argtypes = [i, i, vp, vp, vp, vp, sz, vp, sz, vp, ctypes.POINTER(vp), i, ctypes.POINTER(i)]
     lib.mgs_sort_timed.restype = i
     lib.mgs_sort_path.argtypes = [i, i, sz, vp, sz, vp]
