@@ -176,6 +176,15 @@ __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>
   }
 }
 
+// Clears `words` look-back descriptor words (between the rounds of a pass).
+__global__ void clear_kernel(uint32_t* __restrict__ desc, size_t words, const uint32_t* __restrict__ gate) {
+  if (gated_off(gate)) return;
+  uint4* d4 = reinterpret_cast<uint4*>(desc);
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x, st = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = g; i < words / 4; i += st) d4[i] = make_uint4(0, 0, 0, 0);
+  for (size_t i = words / 4 * 4 + g; i < words; i += st) desc[i] = 0;
+}
+
 // Which bytes vary: bit b set iff byte b differs between keys.
 __device__ __forceinline__ uint32_t varying_bytes(const uint64_t* red, int passes) {
   const uint64_t diff = red[0] ^ red[1];
@@ -244,16 +253,35 @@ __device__ __forceinline__ void load_tile(const typename Codec<KT>::Bits* __rest
   }
 }
 
+// Look-back window of one launch of a pass.  The descriptors of very large
+// sorts would outgrow the scratch budget, so such a pass runs in rounds:
+// round r covers tiles [first_t, first_t + tpr) of every chain, its
+// descriptors occupy slots c * tpr + (tile index inside the round), and the
+// chain's inclusive digit counts carry from one round to the next.  One round
+// (first_t = 0, tpr = tiles per chain) is the plain pass.
+struct PassRound {
+  uint32_t first_t;          // r * tpr
+  uint32_t tpr;              // tiles per chain and round
+  int rounds;                // > 1: descriptors cleared before every launch
+  const uint32_t* carry_in;  // [kChains][256] digit counts before the round (round > 0)
+  uint32_t* carry_out;       // [kChains][256] digit counts after it (all rounds but the last)
+};
+
+// Tiles of chain c inside round rd.
+__device__ __forceinline__ uint32_t round_tiles(const ChainMap& cm, const PassRound& rd, uint32_t c) {
+  const uint32_t ct = min(cm.tpc, cm.tiles - c * cm.tpc);
+  return ct > rd.first_t ? min(rd.tpr, ct - rd.first_t) : 0u;
+}
+
 // Next tile for a workgroup: tiles of its current chain in order, then of the
 // chain `step` further on (modulo the chain count) once that one is used up.
 // Thread-private state: the current chain and the number of chains of its
 // group not yet found empty.
-__device__ __forceinline__ uint32_t take_tile(uint32_t* __restrict__ cnt, const ChainMap& cm, uint32_t& c,
-                                              uint32_t& left, uint32_t step = 1) {
+__device__ __forceinline__ uint32_t take_tile(uint32_t* __restrict__ cnt, const ChainMap& cm, const PassRound& rd,
+                                              uint32_t& c, uint32_t& left, uint32_t step = 1) {
   while (left > 0) {
     const uint32_t t = atomicAdd(&cnt[c], 1u);
-    const uint32_t first = c * cm.tpc;
-    if (t < min(cm.tpc, cm.tiles - first)) return first + t;
+    if (t < round_tiles(cm, rd, c)) return c * cm.tpc + rd.first_t + t;
     c += step;
     if (c >= cm.chains) c -= cm.chains;
     --left;
@@ -335,7 +363,8 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
     const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, ChainMap cm,
     const uint64_t* __restrict__ red, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters,
-    D* __restrict__ desc, uint32_t* __restrict__ status, uint32_t spin_limit, const uint32_t* __restrict__ gate) {
+    D* __restrict__ desc, uint32_t* __restrict__ status, uint32_t spin_limit, const uint32_t* __restrict__ gate,
+    PassRound rd) {
   using Bits = typename Codec<KT>::Bits;
   constexpr int P = (int)sizeof(Bits);
   constexpr int W = BLOCK / kWave;
@@ -370,8 +399,10 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     vsrc = pick_src(sb, v_in, (const V*)v_alt, (const V*)v_out);
     vdst = pick_dst(db, v_alt, v_out);
   }
-  const int shift = 8 * pass, parity = k & 1;
-  uint32_t* __restrict__ cnt = counters + (size_t)pass * kChains;
+  // status rotation by pass (descriptors cleared once per sort); with rounds
+  // the descriptors are cleared before every launch instead
+  const int shift = 8 * pass, parity = rd.rounds > 1 ? 0 : (k & 1);
+  uint32_t* __restrict__ cnt = counters + ((size_t)pass * rd.rounds + rd.first_t / rd.tpr) * kChains;
   const uint32_t* __restrict__ hp = hist + (size_t)pass * kChains * kRadix;
 
   __shared__ Bits keys[TILE];
@@ -431,7 +462,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
       __syncthreads();
       if ((s_skew != 0) != kUni) return;
     }
-    if (tid == taker) s_tile = take_tile(cnt, cm, my_chain, chains_left, chain_step);
+    if (tid == taker) s_tile = take_tile(cnt, cm, rd, my_chain, chains_left, chain_step);
     __syncthreads();
   }
   uint32_t tile = s_tile;
@@ -444,8 +475,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
 
   while (tile != kNoTile) {
     const uint32_t chain = tile / cm.tpc;
-    const uint32_t first = chain * cm.tpc;
-    const uint32_t lt = tile - first;  // tile index inside its chain
+    const uint32_t lt = tile - chain * cm.tpc - rd.first_t;  // tile index inside this round of its chain
     const uint32_t tile_base = tile * (uint32_t)TILE;
     const uint32_t tile_cnt = min((uint32_t)TILE, n - tile_base);
     const bool full = tile_cnt == (uint32_t)TILE;
@@ -488,16 +518,19 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     __syncthreads();
 
     uint32_t total = 0;
-    const D* chain_desc = desc + (size_t)first * kRadix;
+    D* const chain_desc = desc + (size_t)chain * rd.tpr * kRadix;
     D win[kPassWindow];
     uint32_t taken = kNoTile;
     if (lb_wave) {
       const uint32_t d = (uint32_t)tid - kRadix;
-      if (tid == taker) taken = take_tile(cnt, cm, my_chain, chains_left, chain_step);
+      if (tid == taker) taken = take_tile(cnt, cm, rd, my_chain, chains_left, chain_step);
+      // the round's first tile of a chain starts from the counts the
+      // previous round carried over
+      const uint32_t carry = (lt == 0 && rd.carry_in != nullptr) ? rd.carry_in[chain * kRadix + d] : 0u;
 #pragma unroll
       for (int i = 0; i < W; ++i) total += off[i][d];
-      publish(desc + (size_t)tile * kRadix + d, Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)total, parity),
-              xcd_local);
+      publish(chain_desc + (size_t)lt * kRadix + d,
+              Desc<D>::make(lt == 0 ? kStInclusive : kStAggregate, (D)(carry + total), parity), xcd_local);
       if (lt > 0) lookback_load(chain_desc, (int64_t)lt - 1, d, win);
     } else if (tid < kRadix) {
 #pragma unroll
@@ -559,16 +592,17 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     if (lb_wave) {
       // this wavefront's part of the tile is in LDS: finish the look-back
       const uint32_t d = (uint32_t)tid - kRadix;
+      uint32_t prefix = 0;
+      if (lt == 0 && rd.carry_in != nullptr) prefix = rd.carry_in[chain * kRadix + d];
       if (lt > 0) {
         bool gave_up = false;
-        const uint32_t prefix =
-            lb_done ? lb_prefix : lookback_from(chain_desc, lb_t, d, parity, win, lb_prefix, spin_limit, gave_up);
+        prefix = lb_done ? lb_prefix : lookback_from(chain_desc, lb_t, d, parity, win, lb_prefix, spin_limit, gave_up);
         if (gave_up) atomicOr(status, kStatusLookback);
-        publish(desc + (size_t)tile * kRadix + d, Desc<D>::make(kStInclusive, (D)(prefix + total), parity), xcd_local);
-        gbase[d] = s_prefix[d] + prefix;
-      } else {
-        gbase[d] = s_prefix[d];
+        publish(chain_desc + (size_t)lt * kRadix + d, Desc<D>::make(kStInclusive, (D)(prefix + total), parity),
+                xcd_local);
       }
+      gbase[d] = s_prefix[d] + prefix;
+      if (rd.carry_out != nullptr && lt + 1 == round_tiles(cm, rd, chain)) rd.carry_out[chain * kRadix + d] = prefix + total;
     }
     __syncthreads();
 
@@ -612,16 +646,18 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
 template <typename Bits, typename V>
 __global__ void fixup_kernel(const Bits* k_in, const Bits* k_alt, Bits* k_out, const V* v_in, const V* v_alt, V* v_out,
                              uint32_t n, const uint64_t* __restrict__ red, int aliased,
-                             const uint32_t* __restrict__ counters, ChainMap cm, uint32_t* __restrict__ status,
-                             const uint32_t* __restrict__ gate) {
+                             const uint32_t* __restrict__ counters, ChainMap cm, uint32_t tpr, int rounds,
+                             uint32_t* __restrict__ status, const uint32_t* __restrict__ gate) {
   constexpr int P = (int)sizeof(Bits);
   if (gated_off(gate)) return;
   const uint32_t vary = varying_bytes(red, P);
-  if (blockIdx.x == 0 && threadIdx.x < (uint32_t)(P * kChains)) {
-    const uint32_t p = threadIdx.x / kChains, c = threadIdx.x % kChains;
-    if ((vary >> p) & 1u && c < cm.chains) {
-      const uint32_t want = min(cm.tpc, cm.tiles - c * cm.tpc);
-      if (counters[p * kChains + c] < want) atomicOr(status, kStatusChainUnserved);
+  if (blockIdx.x == 0) {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(P * rounds * kChains); i += blockDim.x) {
+      const uint32_t c = i % kChains, r = (i / kChains) % rounds, p = i / (kChains * rounds);
+      if ((vary >> p) & 1u && c < cm.chains) {
+        const PassRound rd{r * tpr, tpr, rounds, nullptr, nullptr};
+        if (counters[i] < round_tiles(cm, rd, c)) atomicOr(status, kStatusChainUnserved);
+      }
     }
   }
   const int na = __builtin_popcount(vary);
@@ -703,9 +739,23 @@ bool rank_peers_requested() {
   return e != nullptr && e[0] == '1' && e[1] == 0;
 }
 
+// Look-back descriptor memory of one sort: at most this many bytes
+// (MGS_DESC_BUDGET, a decimal byte count in the environment, overrides it);
+// a sort whose descriptors would need more runs each pass in rounds.
+size_t desc_budget() {
+  const char* e = std::getenv("MGS_DESC_BUDGET");
+  if (e != nullptr && e[0] != 0) {
+    char* end = nullptr;
+    const unsigned long long v = std::strtoull(e, &end, 10);
+    if (end != nullptr && *end == 0 && v >= 2 * kChains * kRadix * 4) return (size_t)v;
+  }
+  return size_t(96) << 20;
+}
+
 struct SortLayout {
-  size_t status, alt_k, alt_v, hist, red, counters, msd_ctl, desc, total;
+  size_t status, alt_k, alt_v, hist, red, counters, carry, msd_ctl, desc, total;
   uint32_t num_tiles;
+  uint32_t rounds, tpr;  // rounds per pass and tiles per chain and round
   bool wide_desc;
 };
 
@@ -714,9 +764,25 @@ SortLayout sort_layout_cfg(size_t n) {
   constexpr size_t tile = (size_t)PassCfg<KB, VB, BIG>::kTile;
   SortLayout L{};
   L.num_tiles = (uint32_t)((n + tile - 1) / tile);
+  const ChainMap cm = make_chain_map(n, (uint32_t)tile, kChains);
   // descriptor values are prefixes inside one chain: 30 bits hold them
   // unless a chain reaches 2^30 keys
-  L.wide_desc = make_chain_map(n, (uint32_t)tile, kChains).chain_keys >= (size_t(1) << 30);
+  L.wide_desc = cm.chain_keys >= (size_t(1) << 30);
+  const size_t dbytes = L.wide_desc ? 8 : 4;
+  const size_t full = (size_t)cm.chains * cm.tpc * kRadix * dbytes;
+  const size_t budget = desc_budget();
+  L.rounds = 1;
+  L.tpr = cm.tpc;
+  size_t desc_bytes = full;
+  if (full > budget) {
+    // the fewest rounds whose window fits; the full budget is reserved, so
+    // that the scratch size never decreases with n
+    uint32_t r = (uint32_t)((full + budget - 1) / budget);
+    while ((size_t)cm.chains * ((cm.tpc + r - 1) / r) * kRadix * dbytes > budget) ++r;
+    L.rounds = r;
+    L.tpr = (cm.tpc + r - 1) / r;
+    desc_bytes = budget;
+  }
   size_t o = 0;
   auto take = [&](size_t bytes) {
     const size_t at = o;
@@ -728,9 +794,10 @@ SortLayout sort_layout_cfg(size_t n) {
   L.alt_v = take(n * VB);
   L.hist = take((size_t)KB * kChains * kRadix * 4);
   L.red = take(3 * sizeof(uint64_t));
-  L.counters = take((size_t)KB * kChains * 4);
+  L.counters = take((size_t)KB * L.rounds * kChains * 4);
+  L.carry = take(2 * (size_t)kChains * kRadix * 4);
   L.msd_ctl = take(sizeof(MsdCtl));  // the three-pass path's decision word: read by the LSD kernels
-  L.desc = take((size_t)L.num_tiles * kRadix * (L.wide_desc ? 8 : 4));
+  L.desc = take(desc_bytes);
   L.total = o;
   return L;
 }
@@ -816,9 +883,11 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
   uint64_t* red = reinterpret_cast<uint64_t*>(t + L.red);
   uint32_t* counters = reinterpret_cast<uint32_t*>(t + L.counters);
   void* desc = t + L.desc;
-  const size_t desc_words = (size_t)L.num_tiles * kRadix * (L.wide_desc ? 2 : 1);
   const int aliased = (kin == kout) || (kHasVal<V> && vin == vout);
   const ChainMap cm = make_chain_map(n, (uint32_t)Cfg::kTile, kChains);
+  // descriptor slots in use: one per tile, or one per tile of a round
+  const size_t desc_slots = L.rounds > 1 ? (size_t)cm.chains * L.tpr : (size_t)L.num_tiles;
+  const size_t desc_words = desc_slots * kRadix * (L.wide_desc ? 2 : 1);
   const uint32_t spin_limit = lookback_spin_limit();
   const uint32_t* gate = nullptr;
   if constexpr (KB == 4 && VB == 0) {
@@ -830,8 +899,8 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
   }
 
   mark(s, "init");
-  init_kernel<<<1, 1024, 0, s>>>(hist, (uint32_t)(P * kChains * kRadix), counters, (uint32_t)(P * kChains), red, status,
-                                 gate);
+  init_kernel<<<1, 1024, 0, s>>>(hist, (uint32_t)(P * kChains * kRadix), counters, (uint32_t)(P * L.rounds * kChains),
+                                 red, status, gate);
   mark(s, "upfront");
   {
     constexpr int HB = 256;
@@ -846,12 +915,14 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
       upfront_kernel<KT, HB, false><<<grid, HB, 0, s>>>(ki, (uint32_t)n, hist, red, (uint32_t*)desc, desc_words, cm,
                                                         xcd_local_allowed(), gate);
   }
+  uint32_t* carry = reinterpret_cast<uint32_t*>(t + L.carry);
+  PassRound rd{0, L.tpr, (int)L.rounds, nullptr, nullptr};
   auto launch_pass = [&](auto kernel, auto* dsc, int p) {
     const uint32_t res = resident_blocks(kernel, Cfg::kBlock, 0);
     uint32_t grid = std::min<uint32_t>(L.num_tiles, res ? res : L.num_tiles);
     grid = std::max<uint32_t>(grid, cm.chains);  // every chain has a home workgroup
     kernel<<<grid, Cfg::kBlock, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, p, aliased, cm, red, hist, counters,
-                                        dsc, status, spin_limit, gate);
+                                        dsc, status, spin_limit, gate, rd);
   };
   // Large sorts launch each pass as a pair of pass_kernel variants, one-
   // atomic-per-key ranking and uniform-digit ranking; the variant that does
@@ -879,16 +950,26 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
                                                     "pass4", "pass5", "pass6", "pass7"};
   for (int p = 0; p < P; ++p) {
     mark(s, kPassName[p]);
-    if (L.wide_desc)
-      launch_ranked(static_cast<uint64_t*>(desc), p);
-    else
-      launch_ranked(static_cast<uint32_t*>(desc), p);
+    for (uint32_t r = 0; r < L.rounds; ++r) {
+      if (L.rounds > 1) {
+        // every round starts from cleared descriptors (the first round of
+        // the first pass finds them cleared by the upfront kernel)
+        if (p > 0 || r > 0) clear_kernel<<<1024, 256, 0, s>>>(static_cast<uint32_t*>(desc), desc_words, gate);
+        rd.first_t = r * L.tpr;
+        rd.carry_in = r > 0 ? carry + ((r - 1) & 1) * kChains * kRadix : nullptr;
+        rd.carry_out = r + 1 < L.rounds ? carry + (r & 1) * kChains * kRadix : nullptr;
+      }
+      if (L.wide_desc)
+        launch_ranked(static_cast<uint64_t*>(desc), p);
+      else
+        launch_ranked(static_cast<uint32_t*>(desc), p);
+    }
   }
   mark(s, "fixup");
   {
     const int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
-    fixup_kernel<Bits, V><<<grid, 256, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, red, aliased, counters, cm, status,
-                                               gate);
+    fixup_kernel<Bits, V><<<grid, 256, 0, s>>>(ki, ka, ko, vi, va, vo, (uint32_t)n, red, aliased, counters, cm, L.tpr,
+                                               (int)L.rounds, status, gate);
   }
   mark(s);
   return check_launch();

@@ -212,7 +212,8 @@ std::string ArgumentLimits::Usage() {
     << ")\n"
     << "--num_buffers\t\t<num_buffers>\t\tThe number of device buffers [2, 3] (= " << GetDefaultNumBuffers()
     << ")\n"
-    << "--merge_backend\t\t<merge_backend>\t\tFinal merge {cpu, gpu} (= cpu)\n";
+    << "--merge_backend\t\t<merge_backend>\t\tFinal merge {cpu, gpu} (= cpu)\n"
+    << "--block_size\t\t<block_size>\t\tBytes per in-place transfer block of --num_buffers 3 (= chunk/100)\n";
   return u.str();
 }
 

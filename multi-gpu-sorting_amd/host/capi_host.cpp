@@ -17,6 +17,7 @@
 #include "host_api.h"
 #include "host_vector.h"
 #include "key_traits.h"
+#include "device_buffer.h"
 #include "hip_check.h"
 #include "paradis.h"
 #include "sort_server.h"
@@ -112,7 +113,7 @@ int mgh_tanasic_sort(const char* dtype, void* keys, size_t n, int gpu, size_t pa
 }
 
 int mgh_gowanlock_sort(const char* dtype, void* keys, size_t n, int gpu, size_t chunk_size, size_t merge_group_size,
-                       size_t num_buffers, const char* merge_backend, int threads) {
+                       size_t num_buffers, const char* merge_backend, int threads, size_t block_bytes) {
   return Guard([&] {
     return ByType(dtype, [&](auto* tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
@@ -120,7 +121,7 @@ int mgh_gowanlock_sort(const char* dtype, void* keys, size_t n, int gpu, size_t 
       mgsort::SetupGowanlockSort({gpu}, threads);
       return OnHostVector<T>(keys, n, [&](mgsort::HostVector<T>& v) {
         mgsort::GowanlockSort<T>(&v, chunk_size, merge_group_size, num_buffers, {gpu},
-                                 merge_backend ? merge_backend : "cpu");
+                                 merge_backend ? merge_backend : "cpu", block_bytes);
       });
     });
   });
@@ -202,6 +203,75 @@ int mgh_serve(mgh_serve_request* reqs, size_t count, int gpu, int batched, int m
     }
     if (gkeys_s) *gkeys_s = (count && last > first) ? keys / (last - first) / 1e9 : 0.0;
     return rc;
+  });
+}
+
+int mgh_link_probe(int gpu, int mode, size_t bytes, size_t block_bytes, int reps, double* gbps) {
+  if (gbps == nullptr || bytes == 0 || mode < 0 || mode > 3) return -4;
+  return Guard([&] {
+    HIP_CHECK(hipSetDevice(gpu));
+    mgsort::HostVector<uint8_t> h_in(bytes, false), h_out(bytes, false);
+    if (!h_in.pinned() || !h_out.pinned()) throw mgsort::Error("pinned host allocation failed");
+    std::memset(h_in.data(), 1, bytes);
+    std::memset(h_out.data(), 2, bytes);
+    mgsort::DeviceBuffer d(bytes), d2(mode == 2 ? bytes : 1);
+    mgsort::Stream s1, s2;
+    mgsort::Event t0(true), t1(true), ev(false);
+    const size_t block = block_bytes ? std::min(block_bytes, bytes) : bytes;
+    std::vector<std::unique_ptr<mgsort::Event>> out_ev;
+    if (mode == 3)
+      for (size_t at = 0; at < bytes; at += block) out_ev.emplace_back(new mgsort::Event());
+    double best = 1e300;
+    for (int r = 0; r < std::max(1, reps) + 1; ++r) {  // the first repetition warms up
+      HIP_CHECK(hipDeviceSynchronize());
+      t0.Record(s1);
+      HIP_CHECK(hipStreamWaitEvent(s2, t0, 0));
+      char* dp = static_cast<char*>(d.get());
+      if (mode == 0) {
+        HIP_CHECK(hipMemcpyAsync(dp, h_in.data(), bytes, hipMemcpyHostToDevice, s1));
+      } else if (mode == 1) {
+        HIP_CHECK(hipMemcpyAsync(h_out.data(), dp, bytes, hipMemcpyDeviceToHost, s1));
+      } else if (mode == 2) {
+        HIP_CHECK(hipMemcpyAsync(dp, h_in.data(), bytes, hipMemcpyHostToDevice, s1));
+        HIP_CHECK(hipMemcpyAsync(h_out.data(), d2.get(), bytes, hipMemcpyDeviceToHost, s2));
+      } else {
+        size_t k = 0;
+        for (size_t at = 0; at < bytes; at += block, ++k) {
+          const size_t b = std::min(block, bytes - at);
+          HIP_CHECK(hipMemcpyAsync(h_out.data() + at, dp + at, b, hipMemcpyDeviceToHost, s2));
+          out_ev[k]->Record(s2);
+          HIP_CHECK(hipStreamWaitEvent(s1, *out_ev[k], 0));
+          HIP_CHECK(hipMemcpyAsync(dp + at, h_in.data() + at, b, hipMemcpyHostToDevice, s1));
+        }
+      }
+      ev.Record(s2);
+      HIP_CHECK(hipStreamWaitEvent(s1, ev, 0));
+      t1.Record(s1);
+      t1.Sync();
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+      if (r > 0) best = std::min(best, (double)ms);
+    }
+    const double moved = (mode >= 2 ? 2.0 : 1.0) * (double)bytes;
+    *gbps = moved / (best / 1e3) / 1e9;
+    return 0;
+  });
+}
+
+int mgh_multiway_merge(const char* dtype, const void* in, size_t k, size_t run_len, void* out, int threads) {
+  if ((in == nullptr || out == nullptr) && k * run_len > 0) return -4;
+  return Guard([&] {
+    return ByType(dtype, [&](auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      omp_set_num_threads(threads);
+      // the GNU multiway merge takes mutable run iterators; it only reads them
+      T* p = const_cast<T*>(static_cast<const T*>(in));
+      std::vector<std::pair<T*, T*>> runs;
+      for (size_t i = 0; i < k; ++i) runs.push_back({p + i * run_len, p + (i + 1) * run_len});
+      __gnu_parallel::multiway_merge(runs.begin(), runs.end(), static_cast<T*>(out), (std::ptrdiff_t)(k * run_len),
+                                     mgsort::EncodedLess<T>());
+      return 0;
+    });
   });
 }
 

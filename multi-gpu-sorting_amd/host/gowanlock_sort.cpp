@@ -102,11 +102,31 @@ T* MergeRunsOnDevice(std::vector<std::pair<size_t, size_t>> runs, T* a, T* b, hi
 
 }  // namespace
 
+// Keeps the merge thread from outliving an exception: wakes it with every
+// run marked sorted and joins it (its output is discarded with the error).
+struct MergerGuard {
+  std::thread* t;
+  Progress* prog;
+  size_t all;
+  ~MergerGuard() {
+    if (t->joinable()) {
+      {
+        std::lock_guard<std::mutex> lk(prog->mu);
+        prog->sorted = all;
+      }
+      prog->cv.notify_all();
+      t->join();
+    }
+  }
+};
+
 template <typename T>
 void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_group_size, size_t num_buffers,
-                   const std::vector<int>& gpus, const std::string& merge_backend) {
+                   const std::vector<int>& gpus, const std::string& merge_backend, size_t block_bytes) {
   if (elements->empty()) return;
   if (num_buffers < 2 || num_buffers > 3) throw Error("num_buffers must be 2 or 3");
+  if (!elements->pinned())
+    throw Error("the HET sort needs the input in pinned host memory (the pinned allocation failed)");
   const size_t n = elements->size();
   const GowanlockPlan plan = PlanGowanlock(n, chunk_size, merge_group_size, gpus.size());
   const size_t C = plan.num_chunks, cs = plan.chunk_size, mgs = plan.merge_group_size;
@@ -120,20 +140,31 @@ void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_grou
     const size_t need = 2 * n * sizeof(T) + mgs_sort_temp_bytes(kt, MGS_VAL_NONE, cs) + (size_t(1) << 28);
     gpu_merge = need < free_b;
   }
+  const size_t temp_bytes = mgs_sort_temp_bytes(kt, MGS_VAL_NONE, cs);
 
   TimeDurations::Get()->Tic("memory_allocation");
   HostVector<T> secondary;
-  if (!gpu_merge && plan.final_runs > 1) secondary.Resize(n, false);
-  const size_t nb = gpu_merge ? 1 : num_buffers;
-  std::vector<DeviceBuffer> bufs(nb);
-  DeviceBuffer resident, resident_alt;
+  if (!gpu_merge && plan.final_runs > 1) {
+    secondary.Resize(n, false);
+    if (!secondary.pinned()) throw Error("the HET sort's secondary host buffer could not be pinned");
+  }
+  // Device memory plans of the reference (src/gowanlock_sort.cuh:127-393):
+  //   2 buffers  two chunk buffers; the sort of the chunk in one uses the
+  //              other as its scratch (2n)
+  //   3 buffers  two chunk buffers plus the sort's own scratch (3n)
+  //   gpu merge  every run resident, plus the merge's second copy
+  std::vector<DeviceBuffer> bufs(gpu_merge ? 0 : 2);
+  DeviceBuffer resident, resident_alt, sort_temp;
   if (gpu_merge) {
     resident.Allocate(n * sizeof(T));
     resident_alt.Allocate(n * sizeof(T));
+    sort_temp.Allocate(temp_bytes);
+  } else if (num_buffers == 2) {
+    for (auto& b : bufs) b.Allocate(std::max(cs * sizeof(T), temp_bytes));
   } else {
     for (auto& b : bufs) b.Allocate(cs * sizeof(T));
+    sort_temp.Allocate(temp_bytes);
   }
-  DeviceBuffer sort_temp(mgs_sort_temp_bytes(kt, MGS_VAL_NONE, cs));
   Stream h2d, comp, d2h;
   std::vector<std::unique_ptr<Event>> h2d_done, sort_done, d2h_done;
   for (size_t c = 0; c < C; ++c) {
@@ -141,12 +172,25 @@ void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_grou
     sort_done.emplace_back(new Event());
     d2h_done.emplace_back(new Event());
   }
+  // in-place transfer blocks (3 buffers): the reference moves a chunk in 100
+  // blocks (InplaceMemcpy, src/gowanlock_sort.cuh:323)
+  const size_t chunk_bytes = cs * sizeof(T);
+  size_t block = block_bytes ? block_bytes : std::max<size_t>(1, chunk_bytes / 100);
+  block = std::max<size_t>(sizeof(T), block / sizeof(T) * sizeof(T));
+  const size_t max_blocks = (chunk_bytes + block - 1) / block;
+  std::vector<std::unique_ptr<Event>> block_out;
+  if (!gpu_merge && num_buffers == 3)
+    for (size_t k = 0; k < max_blocks; ++k) block_out.emplace_back(new Event());
   TimeDurations::Get()->Toc("memory_allocation");
 
   TimeDurations::Get()->Tic("sort_phase");
   T* primary = elements->data();
+  auto run_dst = [&](size_t c) {
+    return (c < plan.num_chunks_to_merge || plan.final_runs == 1) ? primary + c * cs : secondary.data() + c * cs;
+  };
   Progress prog;
   std::thread merger;
+  MergerGuard guard{&merger, &prog, C};
   if (!gpu_merge && plan.num_merge_groups_to_merge > 0) {
     // MergeChunks: merge each group of mgs sorted runs into `secondary` as
     // soon as its last run has landed
@@ -168,23 +212,86 @@ void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_grou
     });
   }
 
-  // Chunk pipeline: H2D(c) waits until D2H(c - nb) freed its buffer.
-  for (size_t c = 0; c < C; ++c) {
-    const size_t len = chunk_len(c), off = c * cs;
-    T* dbuf = gpu_merge ? resident.get<T>() + off : bufs[c % nb].get<T>();
-    if (!gpu_merge && c >= nb) HIP_CHECK(hipStreamWaitEvent(h2d, *d2h_done[c - nb], 0));
-    HIP_CHECK(hipMemcpyAsync(dbuf, primary + off, len * sizeof(T), hipMemcpyHostToDevice, h2d));
-    h2d_done[c]->Record(h2d);
-    HIP_CHECK(hipStreamWaitEvent(comp, *h2d_done[c], 0));
-    MGS_CHECK(mgs_sort(kt, MGS_VAL_NONE, dbuf, dbuf, nullptr, nullptr, len, sort_temp.get(), sort_temp.bytes(), comp));
-    sort_done[c]->Record(comp);
-    if (!gpu_merge) {
-      T* dst = (c < plan.num_chunks_to_merge || plan.final_runs == 1) ? primary + off : secondary.data() + off;
+  if (gpu_merge) {
+    for (size_t c = 0; c < C; ++c) {
+      const size_t len = chunk_len(c), off = c * cs;
+      T* dbuf = resident.get<T>() + off;
+      HIP_CHECK(hipMemcpyAsync(dbuf, primary + off, len * sizeof(T), hipMemcpyHostToDevice, h2d));
+      h2d_done[c]->Record(h2d);
+      HIP_CHECK(hipStreamWaitEvent(comp, *h2d_done[c], 0));
+      MGS_CHECK(mgs_sort(kt, MGS_VAL_NONE, dbuf, dbuf, nullptr, nullptr, len, sort_temp.get(), sort_temp.bytes(), comp));
+    }
+  } else if (num_buffers == 2) {
+    // GowanlockCore2n (reference :127-230): H2D(c+1) overlaps D2H(c); the
+    // sort of chunk c waits until chunk c-1 has left the other buffer, which
+    // it uses as scratch
+    for (size_t c = 0; c < C; ++c) {
+      const size_t len = chunk_len(c), off = c * cs;
+      DeviceBuffer& X = bufs[c % 2];
+      DeviceBuffer& Y = bufs[(c + 1) % 2];
+      if (c >= 1) HIP_CHECK(hipStreamWaitEvent(h2d, *sort_done[c - 1], 0));  // X was its scratch
+      if (c >= 2) HIP_CHECK(hipStreamWaitEvent(h2d, *d2h_done[c - 2], 0));  // X held chunk c-2
+      HIP_CHECK(hipMemcpyAsync(X.get(), primary + off, len * sizeof(T), hipMemcpyHostToDevice, h2d));
+      h2d_done[c]->Record(h2d);
+      HIP_CHECK(hipStreamWaitEvent(comp, *h2d_done[c], 0));
+      if (c >= 1) HIP_CHECK(hipStreamWaitEvent(comp, *d2h_done[c - 1], 0));
+      MGS_CHECK(mgs_sort(kt, MGS_VAL_NONE, X.get(), X.get(), nullptr, nullptr, len, Y.get(), Y.bytes(), comp));
+      sort_done[c]->Record(comp);
       HIP_CHECK(hipStreamWaitEvent(d2h, *sort_done[c], 0));
-      HIP_CHECK(hipMemcpyAsync(dst, dbuf, len * sizeof(T), hipMemcpyDeviceToHost, d2h));
+      HIP_CHECK(hipMemcpyAsync(run_dst(c), X.get(), len * sizeof(T), hipMemcpyDeviceToHost, d2h));
       d2h_done[c]->Record(d2h);
     }
+  } else {
+    // GowanlockCore3n (reference :232-393) with InplaceMemcpy
+    // (src/inplace_memcpy.cuh:7-69): while chunk c sorts in one buffer, the
+    // other buffer sends sorted chunk c-1 back block by block, and each block
+    // it has sent is refilled with the same block of chunk c+1 -- the D2H of
+    // block k+1 overlaps the H2D of block k, ordered by events, not by host
+    // waits
+    HIP_CHECK(hipMemcpyAsync(bufs[0].get(), primary, chunk_len(0) * sizeof(T), hipMemcpyHostToDevice, h2d));
+    h2d_done[0]->Record(h2d);
+    for (size_t c = 0; c < C; ++c) {
+      DeviceBuffer& X = bufs[c % 2];
+      DeviceBuffer& Y = bufs[(c + 1) % 2];
+      HIP_CHECK(hipStreamWaitEvent(comp, *h2d_done[c], 0));
+      MGS_CHECK(mgs_sort(kt, MGS_VAL_NONE, X.get(), X.get(), nullptr, nullptr, chunk_len(c), sort_temp.get(),
+                         sort_temp.bytes(), comp));
+      sort_done[c]->Record(comp);
+      char* y = static_cast<char*>(Y.get());
+      const size_t out_bytes = c >= 1 ? chunk_len(c - 1) * sizeof(T) : 0;
+      const size_t in_bytes = c + 1 < C ? chunk_len(c + 1) * sizeof(T) : 0;
+      const char* in_src = c + 1 < C ? reinterpret_cast<const char*>(primary + (c + 1) * cs) : nullptr;
+      char* out_dst = c >= 1 ? reinterpret_cast<char*>(run_dst(c - 1)) : nullptr;
+      if (c >= 1) HIP_CHECK(hipStreamWaitEvent(d2h, *sort_done[c - 1], 0));
+      const size_t both = std::min(out_bytes, in_bytes);
+      size_t k = 0;
+      for (size_t at = 0; at < std::max(out_bytes, in_bytes); at += block, ++k) {
+        const size_t ob = at < out_bytes ? std::min(block, out_bytes - at) : 0;
+        const size_t ib = at < in_bytes ? std::min(block, in_bytes - at) : 0;
+        if (ob) {
+          HIP_CHECK(hipMemcpyAsync(out_dst + at, y + at, ob, hipMemcpyDeviceToHost, d2h));
+          block_out[k]->Record(d2h);
+        }
+        if (ib) {
+          // the block's old contents must be out first (where chunk c-1 had them)
+          if (at < both || (ob && at < out_bytes)) HIP_CHECK(hipStreamWaitEvent(h2d, *block_out[k], 0));
+          HIP_CHECK(hipMemcpyAsync(y + at, in_src + at, ib, hipMemcpyHostToDevice, h2d));
+        }
+      }
+      if (c >= 1) d2h_done[c - 1]->Record(d2h);
+      if (c + 1 < C) {
+        // chunk c+1 may be longer than chunk c-1 was: its blocks past the
+        // end of c-1 still must not overtake c-1's last block
+        if (c >= 1) HIP_CHECK(hipStreamWaitEvent(h2d, *d2h_done[c - 1], 0));
+        h2d_done[c + 1]->Record(h2d);
+      }
+    }
+    HIP_CHECK(hipStreamWaitEvent(d2h, *sort_done[C - 1], 0));
+    HIP_CHECK(hipMemcpyAsync(run_dst(C - 1), bufs[(C - 1) % 2].get(), chunk_len(C - 1) * sizeof(T),
+                             hipMemcpyDeviceToHost, d2h));
+    d2h_done[C - 1]->Record(d2h);
   }
+
   if (gpu_merge) {
     comp.Sync();
     TimeDurations::Get()->Tic("merge_phase");
@@ -233,16 +340,16 @@ void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_grou
 }
 
 template void GowanlockSort<int32_t>(HostVector<int32_t>*, size_t, size_t, size_t, const std::vector<int>&,
-                                     const std::string&);
+                                     const std::string&, size_t);
 template void GowanlockSort<int64_t>(HostVector<int64_t>*, size_t, size_t, size_t, const std::vector<int>&,
-                                     const std::string&);
+                                     const std::string&, size_t);
 template void GowanlockSort<uint32_t>(HostVector<uint32_t>*, size_t, size_t, size_t, const std::vector<int>&,
-                                      const std::string&);
+                                      const std::string&, size_t);
 template void GowanlockSort<uint64_t>(HostVector<uint64_t>*, size_t, size_t, size_t, const std::vector<int>&,
-                                      const std::string&);
+                                      const std::string&, size_t);
 template void GowanlockSort<float>(HostVector<float>*, size_t, size_t, size_t, const std::vector<int>&,
-                                   const std::string&);
+                                   const std::string&, size_t);
 template void GowanlockSort<double>(HostVector<double>*, size_t, size_t, size_t, const std::vector<int>&,
-                                    const std::string&);
+                                    const std::string&, size_t);
 
 }  // namespace mgsort

@@ -5,8 +5,11 @@
 //
 // Pinned host chunks of chunk_size keys go H2D, are sorted in HBM and come
 // back D2H; copies and sorts of consecutive chunks overlap on three HIP
-// streams ordered by events (num_buffers device chunk buffers: 2 overlaps
-// H2D(c+1) with D2H(c-1); 3 also overlaps both with sort(c)).  A host thread
+// streams ordered by events.  num_buffers 2 (the reference's 2n plan):
+// two chunk buffers, the sort of one using the other as scratch, H2D(c+1)
+// overlapping D2H(c); num_buffers 3 (3n): two chunk buffers and the sort's
+// scratch, the in-place transfer (D2H of chunk c-1 and H2D of chunk c+1
+// through one buffer, block by block) overlapping the sort of chunk c.  A host thread
 // merges groups of merge_group_size sorted runs with the GNU parallel
 // multiway merge as they arrive (MergeChunks, :29-84), and a final multiway
 // merge combines all runs (MergeMergeGroups, :86-95).
@@ -43,8 +46,10 @@ GowanlockPlan PlanGowanlock(size_t num_elements, size_t chunk_size, size_t merge
 
 void SetupGowanlockSort(const std::vector<int>& gpus, int num_threads);
 
+// block_bytes: block of the in-place transfers of the 3-buffer schedule
+// (0 = a hundredth of a chunk, as the reference's Core3n).
 template <typename T>
 void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_group_size, size_t num_buffers,
-                   const std::vector<int>& gpus, const std::string& merge_backend = "cpu");
+                   const std::vector<int>& gpus, const std::string& merge_backend = "cpu", size_t block_bytes = 0);
 
 }  // namespace mgsort

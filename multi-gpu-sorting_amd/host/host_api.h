@@ -23,8 +23,10 @@ int mgh_gnu_parallel_sort(const char* dtype, void* keys, size_t n, int threads);
 // Drop-in drivers on host arrays (TanasicSort, GowanlockSort), in place;
 // phase times afterwards via mgh_phase_seconds("sort_phase") etc.
 int mgh_tanasic_sort(const char* dtype, void* keys, size_t n, int gpu, size_t partitions, uint64_t* pivot);
+// block_bytes: in-place transfer block of the 3-buffer schedule (0 = a
+// hundredth of a chunk).
 int mgh_gowanlock_sort(const char* dtype, void* keys, size_t n, int gpu, size_t chunk_size, size_t merge_group_size,
-                       size_t num_buffers, const char* merge_backend, int threads);
+                       size_t num_buffers, const char* merge_backend, int threads, size_t block_bytes);
 // Chunk/merge-group plan: {chunk_size, merge_group_size, chunks,
 // groups pre-merged, chunks pre-merged, runs of the final merge}.
 int mgh_gowanlock_plan(size_t n, size_t chunk_size, size_t merge_group_size, size_t out[6]);
@@ -56,6 +58,21 @@ typedef struct mgh_serve_request {
 // completed with MGS_OK.
 int mgh_serve(mgh_serve_request* reqs, size_t count, int gpu, int batched, int max_inflight, int host_data,
               double budget_gib, double* gkeys_s);
+
+// Host link probe on device `gpu` (no reference code is run: the modes mirror
+// src/data_transfer_benchmark.cu:83-173 and the in-place transfer of
+// src/inplace_memcpy.cuh:7-69).  bytes of pinned host memory and of device
+// memory, `reps` timed repetitions (best reported).  mode 0: host to device;
+// 1: device to host; 2: both at once on two streams (GB/s counts both
+// directions); 3: in place -- bytes leave a device buffer to the host while
+// other bytes refill it, block by block on two streams (block_bytes; GB/s
+// counts both directions).  *gbps gets the rate.
+int mgh_link_probe(int gpu, int mode, size_t bytes, size_t block_bytes, int reps, double* gbps);
+
+// __gnu_parallel::multiway_merge of k sorted runs of `run_len` keys each,
+// laid out back to back in `in`, into `out` (reference
+// src/gowanlock_sort.cuh:91-92), with `threads` threads.
+int mgh_multiway_merge(const char* dtype, const void* in, size_t k, size_t run_len, void* out, int threads);
 
 // hipEvent_t helpers (for mgs_sort_timed).
 void* mgh_event_create(void);

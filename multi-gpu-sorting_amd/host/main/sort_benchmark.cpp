@@ -6,7 +6,8 @@
 //   memory_allocation_duration, sort_duration, memory_deallocation_duration,
 //   total_duration
 // followed by added columns:
-//   gkeys_per_s, alg_bytes, hbm_gbps_alg, roofline_frac, cpu_threads, verified
+//   gkeys_per_s, alg_bytes, hbm_gbps_alg, roofline_frac, merge_duration,
+//   cpu_threads, verified
 // The run also verifies its output: sorted by the key order and a
 // permutation of the input (order-independent fingerprint); the reference
 // only checks std::is_sorted and still exits 0 (:127, :223-227) -- this
@@ -67,7 +68,7 @@ struct Options {
   std::vector<int> gpus;
   uint32_t seed = 0;
   int num_threads = 1;
-  size_t chunk_size = 0, merge_group_size = 1, num_buffers = 2, partitions = 0;
+  size_t chunk_size = 0, merge_group_size = 1, num_buffers = 2, partitions = 0, block_size = 0;
 };
 
 // Order-independent fingerprint of a multiset of keys (sum and xor of a
@@ -148,7 +149,8 @@ bool GenerateAndSortData(const Options& o) {
     TanasicSort<T>(&elements, o.gpus, o.partitions);
   } else if (a == "gowanlock") {
     SetupGowanlockSort(o.gpus, o.num_threads);
-    GowanlockSort<T>(&elements, o.chunk_size, o.merge_group_size, o.num_buffers, o.gpus, o.merge_backend);
+    GowanlockSort<T>(&elements, o.chunk_size, o.merge_group_size, o.num_buffers, o.gpus, o.merge_backend,
+                     o.block_size);
   } else if (a == "thrust" || a == "hip") {
     DeviceSort<T>(&elements, o, a == "thrust");
   } else if (a == "paradis") {
@@ -188,7 +190,8 @@ bool GenerateAndSortData(const Options& o) {
   const double gkeys = t > 0 ? o.num_elements / t / 1e9 : 0;
   const double gbps = t > 0 ? alg_bytes / t / 1e9 : 0;
   std::cout << std::setprecision(6) << gkeys << "," << (uint64_t)alg_bytes << "," << gbps << "," << gbps / 8000.0
-            << "," << omp_get_num_procs() << "," << (ok ? 1 : 0) << "\n";
+            << "," << std::setprecision(9) << td->Seconds("merge_phase") << "," << omp_get_num_procs() << ","
+            << (ok ? 1 : 0) << "\n";
   return ok;
 }
 
@@ -236,6 +239,8 @@ int main(int argc, char* argv[]) {
     PrintErrorUsageAndExit("Invalid <partitions> = " + v);
   if (GetOptionValue(args, "--merge_backend", &v) && !ArgumentLimits::ParseMergeBackend(v, &o.merge_backend))
     PrintErrorUsageAndExit("Invalid <merge_backend> = " + v);
+  if (GetOptionValue(args, "--block_size", &v) && !ArgumentLimits::ParseChunkSize(v, &o.block_size))
+    PrintErrorUsageAndExit("Invalid <block_size> = " + v);
 
   bool ok = false;
   try {

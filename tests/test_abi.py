@@ -88,6 +88,21 @@ def test_sort_temp_bytes_never_decrease(kt, vt):
         assert b == 0 or b >= n * (kb + vb)
 
 
+@pytest.mark.parametrize("kt,vt", PAIRS)
+def test_sort_scratch_fits_the_reference_buffer_plan(kt, vt):
+    """mgs_sort's scratch stays within the reference's device buffer plan,
+    N * sizeof(T) + 128,000,000 bytes (src/device_buffers.cuh:30-44), for
+    every key and payload type up to MGS_MAX_KEYS, including the reference's
+    HET chunk of 4.125e9 keys (scripts/settings.py:614): above the
+    descriptor budget the LSD passes run in rounds."""
+    import mgsort
+    L = mgsort.lib
+    kb, vb = mgsort.KEY_BYTES[kt], mgsort.VAL_BYTES[vt]
+    for n in (1 << 24, 1 << 30, (1 << 30) + 1, 1 << 31, 4_125_000_000, mgsort.MAX_KEYS):
+        extra = L.mgs_sort_temp_bytes(kt, vt, n) - n * (kb + vb)
+        assert extra <= 128_000_000, (n, extra)
+
+
 def test_too_many_keys_is_rejected_before_any_device_work():
     """n above MGS_MAX_KEYS (the last tile must end below 2^32: the kernels
     index keys with 32 bits) returns MGS_ERR_TOO_LARGE before touching the

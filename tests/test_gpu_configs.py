@@ -63,7 +63,7 @@ def H(mg):
     L.mgh_generate.argtypes = [c, c, ctypes.c_uint32, vp, sz]
     L.mgh_paradis_sort.argtypes = [c, vp, sz, ctypes.c_int]
     L.mgh_gnu_parallel_sort.argtypes = [c, vp, sz, ctypes.c_int]
-    L.mgh_gowanlock_sort.argtypes = [c, vp, sz, ctypes.c_int, sz, sz, sz, c, ctypes.c_int]
+    L.mgh_gowanlock_sort.argtypes = [c, vp, sz, ctypes.c_int, sz, sz, sz, c, ctypes.c_int, sz]
     L.mgh_last_error.restype = c
     L.mgh_serve.argtypes = [vp, sz, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                             ctypes.POINTER(ctypes.c_double)]
@@ -290,7 +290,7 @@ def test_config3_het_nine_chunks_three_buffers(H, backend):
     n = 8 * chunk + 777  # 9 chunks, the last one short
     a = generate(H, "u32", "uniform", SEED, n)
     k = a.copy()
-    rc = H.mgh_gowanlock_sort(b"u32", k.ctypes.data, n, 0, chunk, 8, 3, backend.encode(), host_threads())
+    rc = H.mgh_gowanlock_sort(b"u32", k.ctypes.data, n, 0, chunk, 8, 3, backend.encode(), host_threads(), 0)
     assert rc == 0, H.mgh_last_error()
     exp, _ = O.sort(O.K_U32, a)
     assert np.array_equal(k, exp)

@@ -31,7 +31,7 @@ def H():
     c, vp, sz = ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t
     L.mgh_generate.argtypes = [c, c, ctypes.c_uint32, vp, sz]
     L.mgh_tanasic_sort.argtypes = [c, vp, sz, ctypes.c_int, sz, ctypes.POINTER(ctypes.c_uint64)]
-    L.mgh_gowanlock_sort.argtypes = [c, vp, sz, ctypes.c_int, sz, sz, sz, c, ctypes.c_int]
+    L.mgh_gowanlock_sort.argtypes = [c, vp, sz, ctypes.c_int, sz, sz, sz, c, ctypes.c_int, sz]
     L.mgh_last_error.restype = c
     L.mgh_phase_seconds.argtypes = [c]
     L.mgh_phase_seconds.restype = ctypes.c_double
@@ -99,16 +99,29 @@ def test_gowanlock(H, backend, chunk, group, buffers, dtype):
     n = 777_777
     a = generate(H, dtype, "uniform", 5 + chunk, n)
     k = a.copy()
-    rc = H.mgh_gowanlock_sort(dtype.encode(), k.ctypes.data, n, 0, chunk, group, buffers, backend.encode(), 8)
+    rc = H.mgh_gowanlock_sort(dtype.encode(), k.ctypes.data, n, 0, chunk, group, buffers, backend.encode(), 8, 0)
     assert rc == 0, H.mgh_last_error()
     expect_sorted_like_oracle(dtype, a, k)
+
+
+@pytest.mark.parametrize("block", [1000, 65536, 0])
+@pytest.mark.parametrize("n,chunk", [(777_777, 100_000), (1 << 20, 1 << 18), (300_001, 299_999)])
+def test_gowanlock_inplace_blocks(H, block, n, chunk):
+    """The 3-buffer schedule moves sorted chunk c-1 out of, and chunk c+1 into,
+    one device buffer block by block (in-place transfer of the reference's
+    Core3n): small, odd and default block sizes, a short last chunk."""
+    a = generate(H, "u32", "uniform", 17 + block, n)
+    k = a.copy()
+    rc = H.mgh_gowanlock_sort(b"u32", k.ctypes.data, n, 0, chunk, 3, 3, b"cpu", 8, block)
+    assert rc == 0, H.mgh_last_error()
+    expect_sorted_like_oracle("u32", a, k)
 
 
 def test_gowanlock_small_and_empty(H):
     for n in (0, 1, 2, 17):
         a = generate(H, "u32", "uniform", 3, n) if n else np.empty(0, np.uint32)
         k = a.copy()
-        assert H.mgh_gowanlock_sort(b"u32", k.ctypes.data if n else None, n, 0, 8, 2, 2, b"cpu", 4) == 0
+        assert H.mgh_gowanlock_sort(b"u32", k.ctypes.data if n else None, n, 0, 8, 2, 2, b"cpu", 4, 0) == 0
         expect_sorted_like_oracle("u32", a, k)
 
 

@@ -313,3 +313,33 @@ def test_three_pass_default_size(mg, dist):
     path = mg.lib.mgs_sort_path(kt, mg.VAL_NONE, n, mg._ptr(temp), temp.numel(), None)
     assert path == expected_path(kt, keys)
     assert np.array_equal(from_dev(dk, np.uint32), ek)
+
+
+@pytest.mark.parametrize("kt,vdt", [(O.K_U32, None), (O.K_F32, np.uint32), (O.K_U64, np.uint32), (O.K_I64, np.uint64)],
+                         ids=["u32", "f32+v32", "u64+v32", "i64+v64"])
+@pytest.mark.parametrize("budget", ["16384", "40000"])
+def test_lsd_rounds(mg, kt, vdt, budget, monkeypatch):
+    """A sort whose look-back descriptors exceed the budget (MGS_DESC_BUDGET,
+    forced small here) runs every pass in rounds over a window of each
+    chain's tiles, carrying the chains' digit counts from round to round:
+    same result as the oracle, with the pair of ranking launches and with
+    agent-scope descriptors too, on one temp buffer reused."""
+    import torch
+    monkeypatch.setenv("MGS_DESC_BUDGET", budget)
+    monkeypatch.setenv("MGS_MSD", "0")
+    rng = np.random.default_rng(int(budget) + kt)
+    n = (1 << 21) + 7777
+    keys = random_keys(kt, n, rng, "few")
+    vals = None if vdt is None else random_vals(vdt, n, rng)
+    ek, ev = O.sort(kt, keys, vals)
+    vt = mg.VAL_NONE if vdt is None else (mg.VAL_U32 if vdt == np.uint32 else mg.VAL_U64)
+    temp = mg.temp_buffer(mg.lib.mgs_sort_temp_bytes(kt, vt, n), "cuda")
+    for pair_min, xcd in (("0", "1"), (str(1 << 26), "0"), (str(1 << 26), "1")):
+        monkeypatch.setenv("MGS_SKEW_PAIR_MIN", pair_min)
+        monkeypatch.setenv("MGS_XCD_LOCAL", xcd)
+        dk, dv = to_dev(keys), to_dev(vals)
+        mg.sort(KEY_NAME[kt], dk, dv, temp=temp)
+        assert mg.temp_status(temp) == 0
+        assert np.array_equal(from_dev(dk, keys.dtype), ek), (pair_min, xcd)
+        if vdt is not None:
+            assert np.array_equal(from_dev(dv, vals.dtype), ev), (pair_min, xcd)
