@@ -72,6 +72,7 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
   // 65536 16-bit counters, two per word: bin b in word b >> 1, half b & 1
   __shared__ uint32_t h[kMsdBins / 2];
   __shared__ uint32_t s_rows[kRadix];
+  __shared__ uint32_t s_runs;
   const int tid = threadIdx.x;
   const uint32_t c = blockIdx.x;
   if (c == 0 && tid == 0) {  // nothing else writes these during this kernel
@@ -99,9 +100,33 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
   __syncthreads();
   const size_t lo = (size_t)c * chain_keys;
   const size_t hi = std::min<size_t>((size_t)n, lo + chain_keys);
-  auto count = [&](uint32_t k) {
-    const uint32_t b = top16(Codec<KT>::enc(k));
-    __hip_atomic_fetch_add(&h[b >> 1], 1u << ((b & 1u) << 4), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  auto add = [&](uint32_t b, uint32_t k) {
+    __hip_atomic_fetch_add(&h[b >> 1], k << ((b & 1u) << 4), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  auto count = [&](uint32_t k) { add(top16(Codec<KT>::enc(k)), 1u); };
+  // four consecutive keys per lane: one add when they share a bin, and one
+  // add for all the lanes whose four keys fall in the first lane's bin (runs
+  // of sorted keys, which would otherwise put every lane of the wavefront on
+  // one LDS counter); such keys are counted in `run_keys`
+  uint32_t run_keys = 0;
+  auto count4 = [&](const uint4& v) {
+    const uint32_t b0 = top16(Codec<KT>::enc(v.x)), b1 = top16(Codec<KT>::enc(v.y));
+    const uint32_t b2 = top16(Codec<KT>::enc(v.z)), b3 = top16(Codec<KT>::enc(v.w));
+    const bool same = (b0 == b1) & (b0 == b2) & (b0 == b3);
+    const bool in = same && b0 == readfirstlane(b0);
+    const uint64_t grp = ballot64(in);
+    const uint32_t k = 4u * (uint32_t)__popcll(grp);
+    if (mbcnt64(ballot64(true)) == 0) run_keys += k;
+    if (in) {
+      if (mbcnt64(grp) == 0) add(b0, k);
+    } else if (same) {
+      add(b0, 4u);
+    } else {
+      add(b0, 1u);
+      add(b1, 1u);
+      add(b2, 1u);
+      add(b3, 1u);
+    }
   };
   if (lo < hi) {
     if constexpr (VEC) {
@@ -115,26 +140,17 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
 #pragma unroll
         for (int u = 0; u < U; ++u) v[u] = kv[i + u * kMsdBlock];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          count(v[u].x);
-          count(v[u].y);
-          count(v[u].z);
-          count(v[u].w);
-        }
+        for (int u = 0; u < U; ++u) count4(v[u]);
       }
-      for (; i < nv; i += kMsdBlock) {
-        const uint4 v = kv[i];
-        count(v.x);
-        count(v.y);
-        count(v.z);
-        count(v.w);
-      }
+      for (; i < nv; i += kMsdBlock) count4(kv[i]);
       for (size_t k = lo + nv * 4 + tid; k < hi; k += kMsdBlock) count(keys[k]);
     } else {
       for (size_t k = lo + tid; k < hi; k += kMsdBlock) count(keys[k]);
     }
   }
+  if (tid == 0) s_runs = 0;
   __syncthreads();
+  if (run_keys) atomicAdd(&s_runs, run_keys);
   // byte-3 histogram of the chain: row r is the 128 words [128 r, 128 r + 128)
   const int lane = tid & (kWave - 1), w = tid / kWave;
   for (int r = w; r < kRadix; r += kMsdBlock / kWave) {
@@ -153,7 +169,7 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
     uint32_t s = s_rows[tid] + s_rows[tid + 64] + s_rows[tid + 128] + s_rows[tid + 192];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
-    if (tid == 0) chain_bad[c] = s != (uint32_t)(hi > lo ? hi - lo : 0) ? 1u : 0u;
+    if (tid == 0) chain_bad[c] = (s != (uint32_t)(hi > lo ? hi - lo : 0) ? 1u : 0u) | (s_runs << 1);
   }
   uint4* dst = reinterpret_cast<uint4*>(slices + (size_t)c * (kMsdBins / 2));
   for (int i = tid; i < kMsdBins / 8; i += kMsdBlock)
@@ -177,12 +193,17 @@ __global__ __launch_bounds__(256) void msd_scan_kernel(const uint32_t* __restric
   const uint32_t b = blockIdx.x, d = threadIdx.x;
   const uint32_t s = b * kRadix + d;
   if (b == 0) {
-    uint32_t m = 0;
+    uint32_t m = 0, runs = 0;
     for (uint32_t c = d; c < chains; c += kRadix) {
-      if (chain_bad[c]) atomicOr(&ctl->off, kOffOverflow);
+      const uint32_t cb = chain_bad[c];
+      if (cb & 1u) atomicOr(&ctl->off, kOffOverflow);
+      runs += cb >> 1;
       m |= 1u << (xcd_of[c] & 31u);
     }
     atomicOr(&ctl->xcd_mask, m);
+    const uint32_t before = block_excl_scan_256(runs, sums, (int)d);
+    if (d == kRadix - 1) ctl->runs = before + runs > n / 16 ? 1u : 0u;
+    __syncthreads();  // every thread has read the scan's partial sums
   }
   // sub-bucket size: the chains' 16-bit counters of bin s
   uint32_t cnt = 0;
@@ -327,6 +348,7 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
   constexpr int TILE = kMsdTile;
   using D = uint32_t;
   if (a.ctl->off) return;
+  const bool runs = a.ctl->runs != 0;
 
   __shared__ uint32_t keys[TILE];
   __shared__ uint32_t off[W][kRadix];
@@ -413,7 +435,10 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
       uint32_t dg[IPT];
 #pragma unroll
       for (int i = 0; i < IPT; ++i) dg[i] = digit_of<KT>(kc[i], a.shift);
-      rank_items_full<IPT, false>(dg, ct, pk);
+      if (runs)
+        rank_items_full<IPT, true>(dg, ct, pk);
+      else
+        rank_items_full<IPT, false>(dg, ct, pk);
     } else {
 #pragma unroll
       for (int i = 0; i < IPT; ++i) {

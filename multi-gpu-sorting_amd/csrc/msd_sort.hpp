@@ -32,11 +32,17 @@ struct MsdCfg {
 // Control block.  off: any set bit turns the path off (a 16-bit counter
 // overflowed, a sub-bucket does not fit the local sort, a bucket is far
 // above the mean); the LSD sort launched after the path runs iff off != 0.
+// runs: the passes rank with the uniform-digit path (one LDS atomic for the
+// lanes that share the first lane's digit), chosen when more than a
+// sixteenth of the keys shared their top 16 bits with the first lane of
+// their wavefront load in the upfront (runs of sorted keys, where most lanes
+// of a wavefront would hit one counter).
 struct MsdCtl {
   uint32_t off;
   uint32_t batches;
   uint32_t batch_next;
   uint32_t xcd_mask;  // XCDs the upfront kernel ran on
+  uint32_t runs;
 };
 
 struct MsdLayout {

@@ -40,6 +40,9 @@ def random_keys(kt: int, n: int, rng: np.random.Generator, dist: str = "uniform"
         k = random_keys(kt, n, rng).astype(np.uint64)
         k = (k & ~np.uint64(0xFFFF)) | np.uint64(0x34) | (rng.integers(0, 3, size=n).astype(np.uint64) << np.uint64(8))
         k = k.astype(bd)
+    elif dist == "runs":  # runs of 4096 keys sharing their top 16 bits, the runs spread over the top byte
+        run = (np.arange(n, dtype=np.uint64) // np.uint64(4096)) * np.uint64(257) & np.uint64(0xFFFF)
+        k = ((run << np.uint64(nbits - 16)) | rng.integers(0, 1 << 16, size=n).astype(np.uint64)).astype(bd)
     elif dist == "lowbits":  # only the low byte varies
         k = (rng.integers(0, 256, size=n).astype(np.uint64) | np.uint64(0x1200)).astype(bd)
     else:

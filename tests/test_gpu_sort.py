@@ -270,14 +270,15 @@ def expected_path(kt, keys):
 
 @pytest.mark.parametrize("kt", MSD_KEYS, ids=lambda k: KEY_NAME[k])
 @pytest.mark.parametrize("n", [4097, 100_003, 1 << 20, (1 << 21) + 777])
-@pytest.mark.parametrize("dist", ["uniform", "sorted", "reverse", "few", "equal", "lowbits"])
+@pytest.mark.parametrize("dist", ["uniform", "sorted", "reverse", "few", "equal", "lowbits", "runs"])
 def test_three_pass_path(mg, kt, n, dist, monkeypatch):
     """4-byte keys alone try the three-pass path (MSD pass on byte 3, pass on
     byte 2 inside each bucket, LDS sort of each batch of 16-bit sub-buckets);
     MGS_MSD_MIN=0 tries it at any size.  Keys whose top 16 bits crowd into
     a few sub-buckets (few, equal, lowbits at the larger sizes) hand over to
-    the LSD passes.  Either way the output is the oracle's, in place and out
-    of place."""
+    the LSD passes.  Runs of keys sharing their top 16 bits (runs, sorted at
+    2^26) switch the passes to the uniform-digit ranking.  Either way the
+    output is the oracle's, in place and out of place."""
     import torch
     monkeypatch.setenv("MGS_MSD_MIN", "0")
     rng = np.random.default_rng(n + 3 * kt)
@@ -297,7 +298,7 @@ def test_three_pass_path(mg, kt, n, dist, monkeypatch):
             assert np.array_equal(from_dev(dk, keys.dtype), keys)
 
 
-@pytest.mark.parametrize("dist", ["uniform", "sorted", "skewed"])
+@pytest.mark.parametrize("dist", ["uniform", "sorted", "reverse", "skewed"])
 def test_three_pass_default_size(mg, dist):
     """From 2^26 keys the path is on by default (no environment)."""
     import torch
