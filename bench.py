@@ -388,34 +388,45 @@ def batched_section():
     return out
 
 
-def serving_section(requests=192):
-    """Poisson stream of mixed requests (log-uniform 2^10..2^26 keys; 50% u32,
-    30% u64, 20% u32+u32 pairs; uniform / nearly sorted) through the serving
-    layer: continuous batching vs the plain per-request path."""
+def run_serve(args, tag):
     exe = os.path.join(ROOT, "multi-gpu-sorting_amd", "bin", "serve_benchmark")
-    p = subprocess.Popen([exe, "--requests", str(requests), "--budget_gib", "64", "--loads", "inf,2,1,0.5"],
-                         stdout=subprocess.PIPE, text=True)
+    p = subprocess.Popen([exe] + args, stdout=subprocess.PIPE, text=True)
     rows = []
     for line in p.stdout:  # progress as it comes
-        log("serve:", line.rstrip())
+        log(f"serve{tag}:", line.rstrip())
         if line.startswith("{"):
             rows.append(json.loads(line))
     rc = p.wait(timeout=900)
     if rc != 0:
-        raise RuntimeError(f"serve_benchmark failed ({rc})")
+        raise RuntimeError(f"serve_benchmark {' '.join(args)} failed ({rc})")
     runs = [x for x in rows if "mode" in x]
-    out = {"workload": f"{requests} requests, sizes round(2^(10+16U)), mt19937_64 seed 7; 64 GiB HBM pool",
-           "runs": runs}
-    ceiling = [x for x in rows if x.get("event") == "ceiling"]
-    if ceiling:
-        out["ceiling"] = ceiling[0]
+    out = {"runs": runs}
+    for x in rows:
+        if x.get("event") in ("generated", "ceiling"):
+            out[x["event"]] = x
     for data in ("device", "host"):
         sat = {x["mode"]: x for x in runs if x["data"] == data and x["load"] == "inf"}
         if "batched" in sat and "plain" in sat:
             out[f"{data}_saturated_speedup_vs_plain"] = round(sat["batched"]["gkeys_s"] / sat["plain"]["gkeys_s"], 3)
             out[f"{data}_saturated_speedup_vs_plain_overlap"] = round(
                 sat["batched"]["gkeys_s"] / sat["plain_overlap"]["gkeys_s"], 3)
-    out["same_as_alone"] = all(x["same_as_alone"] for x in runs)
+    out["matches_host_sort"] = all(x["matches_host_sort"] for x in runs) and out["ceiling"]["matches_host_sort"]
+    return out
+
+
+def serving_section(requests=192):
+    """Poisson streams of mixed requests (50% u32, 30% u64, 20% u32+u32
+    pairs; uniform / nearly sorted) through the serving layer: continuous
+    batching vs the plain per-request path, with every output checked against
+    a host sort.  The mixed stream has sizes log-uniform in 2^10..2^26 keys
+    (most keys sit in requests of 2^21 and more); the small-request stream
+    2^10..2^20 (1024 requests)."""
+    out = {"mixed": run_serve(["--requests", str(requests), "--budget_gib", "64", "--loads", "inf,2,1,0.5"], ""),
+           "small": run_serve(["--requests", "1024", "--max_log2", "20", "--budget_gib", "64", "--data", "device",
+                               "--loads", "inf,1"], " small")}
+    out["mixed"]["workload"] = (f"{requests} requests, sizes round(2^(10+16U)), mt19937_64 seed 7; 64 GiB HBM pool; "
+                                "requests of 2^21 keys and more sorted alone, two sort streams")
+    out["small"]["workload"] = "1024 requests, sizes round(2^(10+10U)), mt19937_64 seed 7; device data"
     return out
 
 

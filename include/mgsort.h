@@ -77,7 +77,10 @@ enum {
  * defaults chosen by measurement on MI355X. */
 typedef struct mgs_tuning {
   int32_t large_variant;    /* tile geometry of the large segments' passes: */
-                            /* 0 = default, 1.. = mgs_segmented_variant()   */
+                            /* 0 = default, 0 <= v <                        */
+                            /* mgs_segmented_variant_count(), see           */
+                            /* mgs_segmented_variant(); others are          */
+                            /* MGS_ERR_INVALID                              */
   int32_t small_max;        /* segments up to this length sort inside one   */
                             /* workgroup's LDS (0 = default)                */
   int32_t medium_max;       /* segments up to this length sort with a single*/
@@ -214,10 +217,21 @@ size_t mgs_segmented_sort_temp_bytes(mgs_key kt, mgs_val vt,
                                      size_t max_total_keys, size_t nseg);
 
 /* The same for calls that pass `tuning` (its max_segment_keys shrinks the
- * look-back descriptors); NULL gives mgs_segmented_sort_temp_bytes. */
+ * look-back descriptors, and its large_variant sets the number of large
+ * tiles); NULL gives mgs_segmented_sort_temp_bytes. */
 size_t mgs_segmented_sort_temp_bytes_tuned(mgs_key kt, mgs_val vt,
                                            size_t max_total_keys, size_t nseg,
                                            const mgs_tuning* tuning);
+
+/* Tile geometries of the large segments' passes (no reference counterpart:
+ * the search space of the batching layer's autotuner).  Variants are
+ * 0 .. mgs_segmented_variant_count() - 1 for every (kt, vt); variant v of
+ * (kt, vt) runs workgroups of *block threads with *items_per_thread keys
+ * each (tiles of their product).  Returns MGS_ERR_INVALID for a variant out
+ * of range. */
+int mgs_segmented_variant_count(void);
+int mgs_segmented_variant(mgs_key kt, mgs_val vt, int variant,
+                          int32_t* block, int32_t* items_per_thread);
 
 int mgs_segmented_sort(mgs_key kt, mgs_val vt,
                        const void* keys_in, void* keys_out,

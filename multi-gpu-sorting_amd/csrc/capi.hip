@@ -114,6 +114,12 @@ size_t mgs_segmented_sort_temp_bytes_tuned(mgs_key kt, mgs_val vt, size_t max_to
   return mgs::segmented_temp_bytes(kt, vt, max_total_keys, nseg, tuning);
 }
 
+int mgs_segmented_variant_count(void) { return mgs::segmented_variant_count(); }
+
+int mgs_segmented_variant(mgs_key kt, mgs_val vt, int variant, int32_t* block, int32_t* items_per_thread) {
+  return mgs::segmented_variant(kt, vt, variant, block, items_per_thread);
+}
+
 int mgs_segmented_sort(mgs_key kt, mgs_val vt, const void* keys_in, void* keys_out, const void* vals_in,
                        void* vals_out, const uint64_t* d_offsets, size_t nseg, size_t max_total_keys,
                        const mgs_tuning* tuning, void* temp, size_t temp_bytes, void* stream) {

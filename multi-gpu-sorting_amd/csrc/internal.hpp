@@ -76,6 +76,8 @@ int swap_dispatch(mgs_key kt, mgs_val vt, void* ak, void* av, void* bk, void* bv
 int fill_max_dispatch(mgs_key kt, void* keys, size_t n, hipStream_t s);
 
 size_t segmented_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total, size_t nseg, const mgs_tuning* tuning);
+int segmented_variant(mgs_key kt, mgs_val vt, int var, int32_t* block, int32_t* ipt);
+int segmented_variant_count();
 // ranges == false: d_bounds holds nseg+1 offsets; true: nseg (begin, end)
 // pairs, with payloads at d_val_begin[s] (NULL: at the keys' begin).
 int segmented_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout,

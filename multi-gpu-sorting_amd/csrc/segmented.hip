@@ -528,12 +528,41 @@ struct SegLayout {
 #define MGS_SEG_IPT_WIDE16 14
 #endif
 
+// Tile geometries of the large segments' passes, chosen per call by
+// mgs_tuning::large_variant (0 = the default; the autotuner times all of
+// them per record width, payload and size):
+//   record   variant 0        variant 1          variant 2
+//   4 B      512 x 16         1024 x 18          512 x 8
+//   8 B      512 x 20         512 x 28           512 x 12
+//   12 B     512 x 20         512 x 16           512 x 12
+//   16 B     512 x 14         512 x 12           512 x 8
+// (threads x items per thread).  Variant 0: 16 items per thread for 4-byte
+// keys (two workgroups per CU); wider records use the LDS of a whole CU, as
+// the plain sort's large tiles do.  Variant 1 of the 4-byte keys is the
+// plain sort's tile; the others trade tile size for occupancy.
+constexpr int kSegVariants = 3;
+template <int KB, int VB, int VAR>
+struct SegLargeGeom {
+  static constexpr int kRec = KB + VB;
+  static constexpr int kBlock = (kRec <= 4 && VAR == 1) ? 1024 : 512;
+  static constexpr int kIpt = kRec <= 4    ? (VAR == 0 ? 16 : VAR == 1 ? 18 : 8)
+                              : kRec <= 8  ? (VAR == 0 ? MGS_SEG_IPT_WIDE : VAR == 1 ? 28 : 12)
+                              : kRec <= 12 ? (VAR == 0 ? MGS_SEG_IPT_WIDE : VAR == 1 ? 16 : 12)
+                                           : (VAR == 0 ? MGS_SEG_IPT_WIDE16 : VAR == 1 ? 12 : 8);
+  static constexpr int kTile = kBlock * kIpt;
+};
+
+template <int KB, int VB>
+inline int seg_large_tile(int var) {
+  switch (var) {
+    case 1: return SegLargeGeom<KB, VB, 1>::kTile;
+    case 2: return SegLargeGeom<KB, VB, 2>::kTile;
+    default: return SegLargeGeom<KB, VB, 0>::kTile;
+  }
+}
+
 template <int KB, int VB>
 struct SegCfg {
-  // tiles of the large segments' passes: 16 items per thread for 4-byte
-  // keys (two workgroups per CU); wider records use MGS_SEG_IPT_WIDE items
-  // and the LDS of a whole CU, as the plain sort's large tiles do
-  using T = TileCfg<KB, VB, 16, MGS_SEG_IPT_WIDE, MGS_SEG_IPT_WIDE, MGS_SEG_IPT_WIDE16, 4, MGS_OS_BLOCK>;
   // one workgroup per medium segment streams it through tiles of this size
   using M = TileCfg<KB, VB, 16, 8, 8, 4, 4, MGS_OS_BLOCK>;
   using S = SmallCfg<KB, VB>;
@@ -546,12 +575,11 @@ struct SegCfg {
 constexpr uint32_t kMinLargeExclusive = 256;
 
 template <int KB, int VB>
-SegLayout seg_layout(size_t max_total, size_t nseg, size_t max_seg) {
-  using C = SegCfg<KB, VB>;
+SegLayout seg_layout(size_t max_total, size_t nseg, size_t max_seg, int large_tile) {
   constexpr int P = KB;
   SegLayout L{};
   L.max_large = (uint32_t)std::min<size_t>(nseg, max_total / (kMinLargeExclusive + 1) + 1);
-  L.tiles_ub = (uint32_t)(max_total / C::T::kTile + L.max_large + 1);
+  L.tiles_ub = (uint32_t)(max_total / (size_t)large_tile + L.max_large + 1);
   // descriptor values are prefixes inside one segment: 30 bits hold them
   // unless a segment may reach 2^30 keys
   L.wide_desc = max_seg >= (size_t(1) << 30);
@@ -598,6 +626,32 @@ inline size_t segment_bound(size_t max_total, const mgs_tuning* tuning) {
   return max_total;
 }
 
+inline int large_variant_of(const mgs_tuning* tuning) { return tuning != nullptr ? tuning->large_variant : 0; }
+
+// The large segments' histogram, scan and passes with the tile geometry of
+// variant VAR.
+template <int KT, int VT, int VAR>
+void seg_large(const typename Codec<KT>::Bits* ki, typename Codec<KT>::Bits* ka, typename Codec<KT>::Bits* ko,
+               const typename ValOf<VT>::T* vi, typename ValOf<VT>::T* va, typename ValOf<VT>::T* vo,
+               const SegTab& tab, const uint32_t* large_list, const uint32_t* tile_start, uint32_t* seg_hist,
+               uint32_t* seg_base, uint32_t* tcount, void* desc, SegCtl* ctl, const SegLayout& L, hipStream_t s) {
+  using V = typename ValOf<VT>::T;
+  constexpr int KB = sizeof(typename Codec<KT>::Bits);
+  constexpr int VB = kValBytes<V>;
+  constexpr int P = KB;
+  using G = SegLargeGeom<KB, VB, VAR>;
+  seg_hist_kernel<KT, 256, G::kTile><<<L.tiles_ub, 256, 0, s>>>(ki, tab, large_list, tile_start, seg_hist, ctl);
+  seg_scan_kernel<P><<<L.max_large, 256, 0, s>>>(seg_hist, seg_base, ctl);
+  for (int p = 0; p < P; ++p) {
+    if (L.wide_desc)
+      seg_pass_kernel<KT, V, G::kBlock, G::kIpt, uint64_t><<<L.tiles_ub, G::kBlock, 0, s>>>(
+          ki, ka, ko, vi, va, vo, tab, large_list, tile_start, seg_base, tcount, (uint64_t*)desc, ctl, &ctl->status, p);
+    else
+      seg_pass_kernel<KT, V, G::kBlock, G::kIpt, uint32_t><<<L.tiles_ub, G::kBlock, 0, s>>>(
+          ki, ka, ko, vi, va, vo, tab, large_list, tile_start, seg_base, tcount, (uint32_t*)desc, ctl, &ctl->status, p);
+  }
+}
+
 template <int KT, int VT>
 int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const uint64_t* bounds, bool ranges,
               const uint64_t* val_begin, size_t nseg, size_t max_total, const mgs_tuning* tuning, void* temp, size_t temp_bytes, hipStream_t s) {
@@ -607,7 +661,6 @@ int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const ui
   constexpr int VB = kValBytes<V>;
   constexpr int P = KB;
   using C = SegCfg<KB, VB>;
-  using TC = typename C::T;
   using SC = typename C::S;
   using VP = typename std::conditional<kHasVal<V>, V, uint32_t>::type;
 
@@ -618,7 +671,10 @@ int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const ui
   }
   medium_max = std::max({medium_max, small_max, kMinLargeExclusive});
   const size_t max_seg = segment_bound(max_total, tuning);
-  const SegLayout L = seg_layout<KB, VB>(max_total, nseg, max_seg);
+  const int var = large_variant_of(tuning);
+  if (var < 0 || var >= kSegVariants) return MGS_ERR_INVALID;
+  const int large_tile = seg_large_tile<KB, VB>(var);
+  const SegLayout L = seg_layout<KB, VB>(max_total, nseg, max_seg, large_tile);
   if (temp == nullptr || temp_bytes < L.total) return MGS_ERR_TEMP_TOO_SMALL;
   char* t = static_cast<char*>(temp);
   Bits* ka = reinterpret_cast<Bits*>(t + L.alt_k);
@@ -642,11 +698,11 @@ int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const ui
 
   if (ranges)
     seg_classify_kernel<true><<<1, kClassifyBlock, 0, s>>>(bounds, val_begin, (uint32_t)nseg, max_total, max_seg, small_max, medium_max,
-                                                           TC::kTile, tab, small_list, medium_list, large_list,
+                                                           (uint32_t)large_tile, tab, small_list, medium_list, large_list,
                                                            tile_start, ctl, tcount);
   else
     seg_classify_kernel<false><<<1, kClassifyBlock, 0, s>>>(bounds, val_begin, (uint32_t)nseg, max_total, max_seg, small_max, medium_max,
-                                                            TC::kTile, tab, small_list, medium_list, large_list,
+                                                            (uint32_t)large_tile, tab, small_list, medium_list, large_list,
                                                             tile_start, ctl, tcount);
   seg_clear_kernel<<<2048, 256, 0, s>>>(seg_hist, P * kRadix, (uint32_t*)desc, kRadix * (L.wide_desc ? 2 : 1), ctl);
   {
@@ -656,15 +712,15 @@ int seg_typed(const void* kin, void* kout, const void* vin, void* vout, const ui
     seg_medium_kernel<KT, V, MC::kBlock, MC::kIpt>
         <<<grid, MC::kBlock, 0, s>>>(ki, ka, ko, vi, (V*)va, vo, tab, medium_list, ctl);
   }
-  seg_hist_kernel<KT, 256, TC::kTile><<<L.tiles_ub, 256, 0, s>>>(ki, tab, large_list, tile_start, seg_hist, ctl);
-  seg_scan_kernel<P><<<L.max_large, 256, 0, s>>>(seg_hist, seg_base, ctl);
-  for (int p = 0; p < P; ++p) {
-    if (L.wide_desc)
-      seg_pass_kernel<KT, V, TC::kBlock, TC::kIpt, uint64_t><<<L.tiles_ub, TC::kBlock, 0, s>>>(
-          ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_base, tcount, (uint64_t*)desc, ctl, &ctl->status, p);
-    else
-      seg_pass_kernel<KT, V, TC::kBlock, TC::kIpt, uint32_t><<<L.tiles_ub, TC::kBlock, 0, s>>>(
-          ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_base, tcount, (uint32_t*)desc, ctl, &ctl->status, p);
+  switch (var) {
+    case 1:
+      seg_large<KT, VT, 1>(ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_hist, seg_base, tcount, desc, ctl, L, s);
+      break;
+    case 2:
+      seg_large<KT, VT, 2>(ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_hist, seg_base, tcount, desc, ctl, L, s);
+      break;
+    default:
+      seg_large<KT, VT, 0>(ki, ka, ko, vi, (V*)va, vo, tab, large_list, tile_start, seg_hist, seg_base, tcount, desc, ctl, L, s);
   }
   return check_launch();
 }
@@ -688,8 +744,26 @@ int seg_by_val(mgs_val vt, const void* kin, void* kout, const void* vin, void* v
 }
 
 template <int KB, int VB>
-size_t seg_bytes(size_t max_total, size_t nseg, size_t max_seg) {
-  return seg_layout<KB, VB>(max_total, nseg, max_seg).total;
+size_t seg_bytes(size_t max_total, size_t nseg, size_t max_seg, int var) {
+  if (var < 0 || var >= kSegVariants) var = 0;
+  return seg_layout<KB, VB>(max_total, nseg, max_seg, seg_large_tile<KB, VB>(var)).total;
+}
+
+template <int KB, int VB>
+void seg_geom(int var, int32_t* block, int32_t* ipt) {
+  switch (var) {
+    case 1:
+      *block = SegLargeGeom<KB, VB, 1>::kBlock;
+      *ipt = SegLargeGeom<KB, VB, 1>::kIpt;
+      break;
+    case 2:
+      *block = SegLargeGeom<KB, VB, 2>::kBlock;
+      *ipt = SegLargeGeom<KB, VB, 2>::kIpt;
+      break;
+    default:
+      *block = SegLargeGeom<KB, VB, 0>::kBlock;
+      *ipt = SegLargeGeom<KB, VB, 0>::kIpt;
+  }
 }
 
 }  // namespace
@@ -698,15 +772,32 @@ size_t segmented_temp_bytes(mgs_key kt, mgs_val vt, size_t max_total, size_t nse
   const int kb = key_bytes(kt), vb = val_bytes(vt);
   if (kb < 0 || vb < 0) return 0;
   const size_t ms = segment_bound(max_total, tuning);
+  const int var = large_variant_of(tuning);
   if (kb == 4) {
-    if (vb == 0) return seg_bytes<4, 0>(max_total, nseg, ms);
-    if (vb == 4) return seg_bytes<4, 4>(max_total, nseg, ms);
-    return seg_bytes<4, 8>(max_total, nseg, ms);
+    if (vb == 0) return seg_bytes<4, 0>(max_total, nseg, ms, var);
+    if (vb == 4) return seg_bytes<4, 4>(max_total, nseg, ms, var);
+    return seg_bytes<4, 8>(max_total, nseg, ms, var);
   }
-  if (vb == 0) return seg_bytes<8, 0>(max_total, nseg, ms);
-  if (vb == 4) return seg_bytes<8, 4>(max_total, nseg, ms);
-  return seg_bytes<8, 8>(max_total, nseg, ms);
+  if (vb == 0) return seg_bytes<8, 0>(max_total, nseg, ms, var);
+  if (vb == 4) return seg_bytes<8, 4>(max_total, nseg, ms, var);
+  return seg_bytes<8, 8>(max_total, nseg, ms, var);
 }
+
+int segmented_variant(mgs_key kt, mgs_val vt, int var, int32_t* block, int32_t* ipt) {
+  const int kb = key_bytes(kt), vb = val_bytes(vt);
+  if (kb < 0 || vb < 0) return MGS_ERR_BAD_TYPE;
+  if (var < 0 || var >= kSegVariants) return MGS_ERR_INVALID;
+  int32_t b = 0, i = 0;
+  if (kb == 4)
+    (vb == 0 ? seg_geom<4, 0> : vb == 4 ? seg_geom<4, 4> : seg_geom<4, 8>)(var, &b, &i);
+  else
+    (vb == 0 ? seg_geom<8, 0> : vb == 4 ? seg_geom<8, 4> : seg_geom<8, 8>)(var, &b, &i);
+  if (block) *block = b;
+  if (ipt) *ipt = i;
+  return MGS_OK;
+}
+
+int segmented_variant_count() { return kSegVariants; }
 
 int segmented_dispatch(mgs_key kt, mgs_val vt, const void* kin, void* kout, const void* vin, void* vout,
                        const uint64_t* d_offsets, bool ranges, const uint64_t* d_val_begin, size_t nseg, size_t max_total, const mgs_tuning* tuning,

@@ -5,6 +5,7 @@
 #include <omp.h>
 
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <parallel/algorithm>
@@ -20,6 +21,7 @@
 #include "device_buffer.h"
 #include "hip_check.h"
 #include "paradis.h"
+#include "seg_tuning.h"
 #include "sort_server.h"
 #include "tanasic_sort.h"
 #include "time_durations.h"
@@ -150,6 +152,7 @@ int mgh_serve(mgh_serve_request* reqs, size_t count, int gpu, int batched, int m
     cfg.max_inflight = max_inflight > 0 ? max_inflight : 1;
     cfg.host_data = host_data != 0;
     cfg.hbm_budget = (size_t)(budget_gib * (double)(size_t(1) << 30));
+    if (const char* t = std::getenv("MGS_SEG_TUNING")) cfg.seg_tuning_path = t;
     SortServer server(cfg);
     auto kb = [](int kt) { return (kt == MGS_KEY_U64 || kt == MGS_KEY_I64 || kt == MGS_KEY_F64) ? size_t(8) : size_t(4); };
     auto vb = [](int vt) { return vt == MGS_VAL_U32 ? size_t(4) : vt == MGS_VAL_U64 ? size_t(8) : size_t(0); };
@@ -204,6 +207,20 @@ int mgh_serve(mgh_serve_request* reqs, size_t count, int gpu, int batched, int m
     if (gkeys_s) *gkeys_s = (count && last > first) ? keys / (last - first) / 1e9 : 0.0;
     return rc;
   });
+}
+
+int mgh_seg_tuning_lookup(const char* path, int key_type, int val_type, size_t total_keys, mgs_tuning* out) {
+  if (path == nullptr || out == nullptr) return -4;
+  try {
+    const mgsort::SegTuningTable t = mgsort::SegTuningTable::Load(path);
+    const mgs_tuning* e = t.Lookup((mgs_key)key_type, (mgs_val)val_type, total_keys);
+    if (e == nullptr) return 1;
+    *out = *e;
+    return 0;
+  } catch (const std::exception& ex) {
+    g_error = ex.what();
+    return -4;
+  }
 }
 
 int mgh_link_probe(int gpu, int mode, size_t bytes, size_t block_bytes, int reps, double* gbps) {

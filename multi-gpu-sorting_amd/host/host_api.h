@@ -8,6 +8,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "mgsort.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -73,6 +75,12 @@ int mgh_link_probe(int gpu, int mode, size_t bytes, size_t block_bytes, int reps
 // laid out back to back in `in`, into `out` (reference
 // src/gowanlock_sort.cuh:91-92), with `threads` threads.
 int mgh_multiway_merge(const char* dtype, const void* in, size_t k, size_t run_len, void* out, int threads);
+
+// The segmented sort's tuning table (written by mgsort.tuning; no reference
+// counterpart): loads `path` and writes the entry of the nearest size bin for
+// (key_type, val_type) to *out.  Returns 0, 1 when the table has no entry for
+// the pair (*out untouched), or -4 for a missing or malformed file.
+int mgh_seg_tuning_lookup(const char* path, int key_type, int val_type, size_t total_keys, mgs_tuning* out);
 
 // hipEvent_t helpers (for mgs_sort_timed).
 void* mgh_event_create(void);

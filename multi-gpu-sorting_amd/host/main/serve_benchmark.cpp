@@ -7,20 +7,33 @@
 // index, so stability is observable); each over the reference generator's
 // uniform distribution or sorted-then-1%-swapped ("nearly-sorted-swaps").
 //
-// For every data mode (host: arrays in pinned host memory, copied in and out;
-// device: arrays already in the server's HBM pool, sorted in place) and every
-// load (requests/s as a multiple of the plain path's saturated rate, "inf" =
-// all submitted at once) it prints one JSON line with throughput (GKeys/s),
-// p50/p99/mean latency (submit -> completion event) and whether every output
-// equals the output of the same request sorted alone.
+// The expected output of every request is computed on the host first
+// (std::sort of (key, input index) pairs: the stable order by key, which for
+// keys alone is also the exact multiset check).  Then:
+//   ceiling   each request alone on the device (device data, plain path, one
+//             at a time): total keys over the sum of those times, the rate
+//             the requests would reach one after another at their sizes'
+//             best single-request rate
+//   sweep     for every data mode (host: arrays in pinned host memory,
+//             copied in and out; device: arrays already in the server's HBM
+//             pool, sorted in place) and every load (requests/s as a
+//             multiple of the plain path's saturated rate, "inf" = all
+//             submitted at once), one JSON line per mode with throughput
+//             (GKeys/s), its fraction of the ceiling, p50/p99/mean latency
+//             (submit -> completion event) and whether every output equals
+//             the host-sorted one.
 #include <hip/hip_runtime.h>
 #include <omp.h>
+
+#include <parallel/algorithm>
+#include <unistd.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <random>
 #include <string>
 #include <thread>
@@ -96,22 +109,43 @@ void Generate(const Spec& s, Data& d) {
   }
 }
 
-// Sorted by key, and for pairs ties in payload (= input index) order.
+// Expected output: the (key, input index) pairs sorted, i.e. the stable
+// order by key (unsigned keys: the generator's u32 / u64).
 template <typename K>
-bool CheckSorted(const K* k, const uint32_t* v, size_t n) {
-  bool ok = true;
-#pragma omp parallel for reduction(&& : ok)
-  for (size_t i = 1; i < n; ++i) {
-    const bool in_order = k[i - 1] < k[i] || (k[i - 1] == k[i] && (v == nullptr || v[i - 1] < v[i]));
-    ok = ok && in_order;
+void HostSort(const Spec& s, Data& d) {
+  const K* k = reinterpret_cast<const K*>(d.keys.data());
+  std::vector<std::pair<K, uint32_t>> p(s.n);
+  for (size_t i = 0; i < s.n; ++i) p[i] = {k[i], (uint32_t)i};
+  if (s.n >= (size_t(1) << 20))
+    __gnu_parallel::sort(p.begin(), p.end());
+  else
+    std::sort(p.begin(), p.end());
+  K* ek = reinterpret_cast<K*>(d.exp_keys.data());
+  uint32_t* ev = s.vt ? reinterpret_cast<uint32_t*>(d.exp_vals.data()) : nullptr;
+  for (size_t i = 0; i < s.n; ++i) {
+    ek[i] = p[i].first;
+    if (ev) ev[i] = reinterpret_cast<const uint32_t*>(d.vals.data())[p[i].second];
   }
-  return ok;
+}
+
+void Expect(const Spec& s, Data& d) {
+  if (s.kt == MGS_KEY_U64)
+    HostSort<uint64_t>(s, d);
+  else
+    HostSort<uint32_t>(s, d);
 }
 
 struct RunResult {
-  double wall = 0, p50 = 0, p99 = 0, mean = 0;
+  double wall = 0, p50 = 0, p99 = 0, mean = 0, sum_latency = 0;
   uint64_t keys = 0, batches = 0, failed = 0;
   bool verified = true;
+};
+
+struct Options {
+  size_t budget = size_t(64) << 30;
+  size_t solo_min_keys = size_t(1) << 21;
+  int sort_streams = 2;
+  std::string seg_tuning;
 };
 
 double Percentile(std::vector<double> v, double q) {
@@ -121,15 +155,18 @@ double Percentile(std::vector<double> v, double q) {
   return v[i];
 }
 
-// record_expected: one request at a time (each sorted alone), keeping its
-// output as the expected output of later runs.
+// one_at_a_time: each request is submitted after the previous one completed
+// (each sorted alone).
 RunResult Run(const std::vector<Spec>& specs, std::vector<Data>& data, bool batched, int inflight, bool host_data,
-              double lambda, size_t budget, uint64_t seed, bool record_expected) {
+              double lambda, const Options& opt, uint64_t seed, bool one_at_a_time) {
   ServerConfig cfg;
   cfg.batched = batched;
   cfg.max_inflight = inflight;
   cfg.host_data = host_data;
-  cfg.hbm_budget = budget;
+  cfg.hbm_budget = opt.budget;
+  cfg.solo_min_keys = opt.solo_min_keys;
+  cfg.sort_streams = batched ? opt.sort_streams : 1;
+  cfg.seg_tuning_path = opt.seg_tuning;
   SortServer server(cfg);
   const size_t R = specs.size();
   std::vector<SortRequest> reqs(R);
@@ -170,7 +207,7 @@ RunResult Run(const std::vector<Spec>& specs, std::vector<Data>& data, bool batc
     const double wait = t0 + at[i] - SortServer::Now();
     if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
     server.Submit(&reqs[i]);
-    if (record_expected) server.Drain();
+    if (one_at_a_time) server.Drain();
   }
   server.Drain();
   RunResult res;
@@ -190,6 +227,7 @@ RunResult Run(const std::vector<Spec>& specs, std::vector<Data>& data, bool batc
   res.p99 = Percentile(lat, 0.99) * 1e3;
   double sum = 0;
   for (double x : lat) sum += x;
+  res.sum_latency = sum;
   res.mean = sum / std::max<size_t>(1, lat.size()) * 1e3;
   // results: device mode reads them back from the pool
   for (size_t i = 0; i < R; ++i) {
@@ -202,21 +240,24 @@ RunResult Run(const std::vector<Spec>& specs, std::vector<Data>& data, bool batc
       server.FreeSlot(slots[i]);
     }
     const size_t kb = s.n * KB(s.kt), vb = s.n * VB(s.vt);
-    if (record_expected) {
-      std::memcpy(data[i].exp_keys.data(), data[i].out_keys.data(), kb);
-      if (vb) std::memcpy(data[i].exp_vals.data(), data[i].out_vals.data(), vb);
-      const uint32_t* v = s.vt ? reinterpret_cast<const uint32_t*>(data[i].out_vals.data()) : nullptr;
-      const bool sorted = s.kt == MGS_KEY_U64
-                              ? CheckSorted(reinterpret_cast<const uint64_t*>(data[i].out_keys.data()), v, s.n)
-                              : CheckSorted(reinterpret_cast<const uint32_t*>(data[i].out_keys.data()), v, s.n);
-      res.verified = res.verified && sorted && reqs[i].status == MGS_OK;
-    } else {
-      res.verified = res.verified && reqs[i].status == MGS_OK &&
-                     std::memcmp(data[i].exp_keys.data(), data[i].out_keys.data(), kb) == 0 &&
-                     (vb == 0 || std::memcmp(data[i].exp_vals.data(), data[i].out_vals.data(), vb) == 0);
-    }
+    res.verified = res.verified && reqs[i].status == MGS_OK &&
+                   std::memcmp(data[i].exp_keys.data(), data[i].out_keys.data(), kb) == 0 &&
+                   (vb == 0 || std::memcmp(data[i].exp_vals.data(), data[i].out_vals.data(), vb) == 0);
   }
   return res;
+}
+
+// The package's tuning table beside this binary's directory
+// (bin/../mgsort/seg_tuning.json), if present.
+std::string DefaultTuningPath() {
+  char buf[4096];
+  const ssize_t len = readlink("/proc/self/exe", buf, sizeof(buf) - 1);
+  if (len <= 0) return "";
+  buf[len] = 0;
+  std::string dir(buf);
+  dir = dir.substr(0, dir.rfind('/'));
+  const std::string p = dir + "/../mgsort/seg_tuning.json";
+  return std::ifstream(p).good() ? p : "";
 }
 
 bool Arg(int argc, char** argv, const char* name, std::string* v) {
@@ -248,14 +289,23 @@ int main(int argc, char** argv) {
   double lo = 10, hi = 26, budget_gib = 64;
   uint64_t seed = 7;
   std::string data_modes = "device,host", loads = "inf,2,1,0.5";
+  Options opt;
+  opt.seg_tuning = DefaultTuningPath();
   if (Arg(argc, argv, "--requests", &v)) requests = std::stoi(v);
+  // requests of at least 2^solo_min_log2 keys are sorted alone (-1: never)
+  if (Arg(argc, argv, "--solo_min_log2", &v)) {
+    const int e = std::stoi(v);
+    opt.solo_min_keys = e < 0 ? 0 : size_t(1) << e;
+  }
+  if (Arg(argc, argv, "--sort_streams", &v)) opt.sort_streams = std::stoi(v);
+  if (Arg(argc, argv, "--seg_tuning", &v)) opt.seg_tuning = v == "none" ? "" : v;
   if (Arg(argc, argv, "--min_log2", &v)) lo = std::stod(v);
   if (Arg(argc, argv, "--max_log2", &v)) hi = std::stod(v);
   if (Arg(argc, argv, "--budget_gib", &v)) budget_gib = std::stod(v);
   if (Arg(argc, argv, "--seed", &v)) seed = std::stoull(v);
   if (Arg(argc, argv, "--data", &v)) data_modes = v;
   if (Arg(argc, argv, "--loads", &v)) loads = v;
-  const size_t budget = (size_t)(budget_gib * (1ull << 30));
+  opt.budget = (size_t)(budget_gib * (1ull << 30));
 
   try {
     HIP_CHECK(hipSetDevice(0));
@@ -266,14 +316,27 @@ int main(int argc, char** argv) {
       Generate(specs[i], data[i]);
       total += specs[i].n;
     }
-    std::printf("{\"event\": \"generated\", \"requests\": %d, \"keys\": %llu, \"threads\": %d}\n", requests,
-                (unsigned long long)total, omp_get_max_threads());
+    std::printf("{\"event\": \"generated\", \"requests\": %d, \"keys\": %llu, \"threads\": %d, "
+                "\"solo_min_keys\": %zu, \"sort_streams\": %d, \"seg_tuning\": \"%s\"}\n", requests,
+                (unsigned long long)total, omp_get_max_threads(), opt.solo_min_keys, opt.sort_streams,
+                opt.seg_tuning.c_str());
     std::fflush(stdout);
-    // each request alone: one at a time through the plain path (the outputs
-    // every later run must reproduce)
     {
-      const RunResult r = Run(specs, data, false, 1, true, INFINITY, budget, seed, true);
-      std::printf("{\"event\": \"alone\", \"sorted_and_stable\": %s}\n", r.verified ? "true" : "false");
+      const double t0 = SortServer::Now();
+      for (size_t i = 0; i < specs.size(); ++i) Expect(specs[i], data[i]);
+      std::printf("{\"event\": \"expected\", \"what\": \"host sort of (key, index) pairs\", \"s\": %.3f}\n",
+                  SortServer::Now() - t0);
+      std::fflush(stdout);
+    }
+    // each request alone on the device, one at a time through the plain path
+    double ceiling = 0;
+    {
+      const RunResult r = Run(specs, data, false, 1, false, INFINITY, opt, seed, true);
+      ceiling = total / r.sum_latency / 1e9;
+      std::printf("{\"event\": \"ceiling\", \"what\": \"each request alone on the device (data in HBM, mgs_sort, one "
+                  "at a time): total keys / sum of their latencies\", \"alone_sum_s\": %.6f, \"gkeys_s\": %.4f, "
+                  "\"matches_host_sort\": %s}\n",
+                  r.sum_latency, ceiling, r.verified ? "true" : "false");
       std::fflush(stdout);
       if (!r.verified) return 1;
     }
@@ -281,7 +344,7 @@ int main(int argc, char** argv) {
     for (const std::string& dm : Split(data_modes)) {
       const bool host = dm == "host";
       // saturated plain rate fixes the request rates of the sweep
-      const RunResult sat = Run(specs, data, false, 1, host, INFINITY, budget, seed, false);
+      const RunResult sat = Run(specs, data, false, 1, host, INFINITY, opt, seed, false);
       const double plain_rps = specs.size() / sat.wall;
       struct Mode {
         const char* name;
@@ -295,14 +358,15 @@ int main(int argc, char** argv) {
         for (const Mode& m : modes) {
           const RunResult r = (inf && !m.batched && m.inflight == 1)
                                   ? sat
-                                  : Run(specs, data, m.batched, m.inflight, host, lambda, budget, seed + 1, false);
+                                  : Run(specs, data, m.batched, m.inflight, host, lambda, opt, seed + 1, false);
           all_ok = all_ok && r.verified;
+          const double gk = r.keys / r.wall / 1e9;
           std::printf(
               "{\"data\": \"%s\", \"mode\": \"%s\", \"load\": \"%s\", \"lambda_rps\": %s, \"requests\": %zu, "
-              "\"keys\": %llu, \"wall_s\": %.6f, \"gkeys_s\": %.4f, \"p50_ms\": %.3f, \"p99_ms\": %.3f, "
-              "\"mean_ms\": %.3f, \"batches\": %llu, \"failed\": %llu, \"same_as_alone\": %s}\n",
+              "\"keys\": %llu, \"wall_s\": %.6f, \"gkeys_s\": %.4f, \"frac_of_ceiling\": %.4f, \"p50_ms\": %.3f, "
+              "\"p99_ms\": %.3f, \"mean_ms\": %.3f, \"batches\": %llu, \"failed\": %llu, \"matches_host_sort\": %s}\n",
               dm.c_str(), m.name, ld.c_str(), inf ? "null" : std::to_string(lambda).c_str(), specs.size(),
-              (unsigned long long)r.keys, r.wall, r.keys / r.wall / 1e9, r.p50, r.p99, r.mean,
+              (unsigned long long)r.keys, r.wall, gk, gk / ceiling, r.p50, r.p99, r.mean,
               (unsigned long long)r.batches, (unsigned long long)r.failed, r.verified ? "true" : "false");
           std::fflush(stdout);
         }

@@ -29,6 +29,8 @@ struct SortServer::Batch {
   int ring = -1;
   size_t keys = 0;
   bool solo = false;  // one request, sorted with mgs_sort
+  bool tuned = false;  // segmented launch with `tuning` (else the defaults)
+  mgs_tuning tuning{};
   hipEvent_t ev_in = nullptr, ev_sort = nullptr, ev_out = nullptr;
 };
 
@@ -47,10 +49,13 @@ void* SortServer::SlotVals(void* slot, mgs_key kt, size_t n) {
 SortServer::SortServer(const ServerConfig& cfg) : cfg_(cfg) {
   if (cfg_.max_inflight < 1) cfg_.max_inflight = 1;
   if (!cfg_.batched) cfg_.max_batch_requests = 1;
+  if (cfg_.sort_streams < 1) cfg_.sort_streams = 1;
+  tuning_ = SegTuningTable::Load(cfg_.seg_tuning_path);
   HIP_CHECK(hipSetDevice(cfg_.device));
   pool_ = std::make_unique<DevicePool>(cfg_.hbm_budget);
   HIP_CHECK(hipStreamCreateWithFlags(&s_in_, hipStreamNonBlocking));
-  HIP_CHECK(hipStreamCreateWithFlags(&s_sort_, hipStreamNonBlocking));
+  s_sort_.assign(cfg_.sort_streams, nullptr);
+  for (auto& st : s_sort_) HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   HIP_CHECK(hipStreamCreateWithFlags(&s_out_, hipStreamNonBlocking));
   h_bounds_.resize(cfg_.max_inflight, nullptr);
   bounds_busy_.assign(cfg_.max_inflight, false);
@@ -72,7 +77,7 @@ SortServer::~SortServer() {
   done_.join();
   for (auto* h : h_bounds_) (void)hipHostFree(h);
   (void)hipStreamDestroy(s_in_);
-  (void)hipStreamDestroy(s_sort_);
+  for (auto st : s_sort_) (void)hipStreamDestroy(st);
   (void)hipStreamDestroy(s_out_);
 }
 
@@ -144,8 +149,11 @@ std::unique_ptr<SortServer::Batch> SortServer::FormBatch(std::unique_lock<std::m
   };
   while (!b->reqs.empty()) {
     const size_t nseg = b->reqs.size();
+    const mgs_tuning* tn = b->solo ? nullptr : tuning_.Lookup(b->kt, b->vt, b->keys);
+    b->tuned = tn != nullptr;
+    if (tn) b->tuning = *tn;
     b->temp_bytes = b->solo ? mgs_sort_temp_bytes(b->kt, b->vt, b->keys)
-                            : mgs_segmented_sort_temp_bytes(b->kt, b->vt, b->keys, nseg);
+                            : mgs_segmented_sort_temp_bytes_tuned(b->kt, b->vt, b->keys, nseg, tn);
     b->scratch = pool_->Allocate(AlignUp(b->temp_bytes) + 3 * nseg * sizeof(uint64_t));
     if (b->scratch != nullptr) break;
     release_from(nseg / 2);  // smaller batch, smaller scratch
@@ -187,8 +195,9 @@ void SortServer::Launch(Batch& b) {
                                       hipMemcpyHostToDevice, s_in_));
     }
   }
+  hipStream_t s_sort = s_sort_[launches_++ % s_sort_.size()];
   HIP_CHECK(hipEventRecord(b.ev_in, s_in_));
-  HIP_CHECK(hipStreamWaitEvent(s_sort_, b.ev_in, 0));
+  HIP_CHECK(hipStreamWaitEvent(s_sort, b.ev_in, 0));
   char* base = pool_->base();
   int st = MGS_OK;
   if (!b.solo) {
@@ -199,16 +208,16 @@ void SortServer::Launch(Batch& b) {
       h[2 * i + 1] = kb + b.reqs[i]->n;
       h[2 * nseg + i] = V ? (uint64_t)(static_cast<char*>(SlotVals(b.slots[i], b.kt, b.reqs[i]->n)) - base) / V : 0;
     }
-    HIP_CHECK(hipMemcpyAsync(b.d_bounds, h, 3 * nseg * sizeof(uint64_t), hipMemcpyHostToDevice, s_sort_));
+    HIP_CHECK(hipMemcpyAsync(b.d_bounds, h, 3 * nseg * sizeof(uint64_t), hipMemcpyHostToDevice, s_sort));
     st = mgs_segmented_sort_ranges(b.kt, b.vt, base, V ? base : nullptr, b.d_bounds, V ? b.d_bounds + 2 * nseg : nullptr,
-                                   nseg, b.keys, nullptr, b.scratch, b.temp_bytes, s_sort_);
+                                   nseg, b.keys, b.tuned ? &b.tuning : nullptr, b.scratch, b.temp_bytes, s_sort);
   } else {
     SortRequest* r = b.reqs[0];
     void* vals = V ? SlotVals(b.slots[0], b.kt, r->n) : nullptr;
-    st = mgs_sort(b.kt, b.vt, b.slots[0], b.slots[0], vals, vals, r->n, b.scratch, b.temp_bytes, s_sort_);
+    st = mgs_sort(b.kt, b.vt, b.slots[0], b.slots[0], vals, vals, r->n, b.scratch, b.temp_bytes, s_sort);
   }
   for (auto* r : b.reqs) r->status = st;
-  HIP_CHECK(hipEventRecord(b.ev_sort, s_sort_));
+  HIP_CHECK(hipEventRecord(b.ev_sort, s_sort));
   HIP_CHECK(hipStreamWaitEvent(s_out_, b.ev_sort, 0));
   if (cfg_.host_data) {
     for (size_t i = 0; i < nseg; ++i) {

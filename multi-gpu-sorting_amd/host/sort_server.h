@@ -9,7 +9,12 @@
 //               them, in place in their pool slots
 //   overlap     up to max_inflight batches are on the device at once: host to
 //               device copies of batch b+1, the sort of batch b and the copies
-//               back of batch b-1 run on three streams ordered by events
+//               back of batch b-1 run on their own streams ordered by events;
+//               consecutive batches sort on sort_streams streams in turn, so
+//               a batch that leaves CUs idle (a mid-size solo sort) overlaps
+//               the next one
+//   tuning      segmented launches take the tuning table's entry for their
+//               type and size (seg_tuning_path; empty: library defaults)
 //   large       in batched mode a request of at least solo_min_keys keys is
 //               sorted alone with mgs_sort: the plain sort already runs at
 //               its per-type rate there, and a batch holding it would make
@@ -29,11 +34,13 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
 #include "device_pool.h"
 #include "mgsort.h"
+#include "seg_tuning.h"
 
 namespace mgsort {
 
@@ -63,6 +70,8 @@ struct ServerConfig {
   size_t max_batch_keys = size_t(1) << 30;
   size_t max_batch_requests = 4096;
   size_t solo_min_keys = size_t(1) << 21;  // 0: batch every request
+  int sort_streams = 2;
+  std::string seg_tuning_path;
 };
 
 struct ServerStats {
@@ -101,7 +110,10 @@ class SortServer {
 
   ServerConfig cfg_;
   std::unique_ptr<DevicePool> pool_;
-  hipStream_t s_in_ = nullptr, s_sort_ = nullptr, s_out_ = nullptr;
+  SegTuningTable tuning_;
+  hipStream_t s_in_ = nullptr, s_out_ = nullptr;
+  std::vector<hipStream_t> s_sort_;
+  uint64_t launches_ = 0;
 
   std::mutex mu_;
   std::condition_variable cv_sched_, cv_done_, cv_drain_;

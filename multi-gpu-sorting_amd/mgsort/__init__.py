@@ -31,8 +31,8 @@ PATH_SMALL, PATH_LSD, PATH_THREE_PASS = 1, 2, 3
 SYMBOLS = [
     "mgs_sort_temp_bytes", "mgs_sort", "mgs_sort_noalloc", "mgs_sort_timed", "mgs_sort_timed_stages", "mgs_sort_path", "mgs_merge_temp_bytes", "mgs_merge", "mgs_select_pivot",
     "mgs_swap_ranges", "mgs_fill_max", "mgs_segmented_sort_temp_bytes", "mgs_segmented_sort_temp_bytes_tuned",
-    "mgs_segmented_sort", "mgs_segmented_sort_ranges", "mgs_temp_status", "mgs_status_str", "mgs_last_hip_error",
-    "mgs_version",
+    "mgs_segmented_sort", "mgs_segmented_sort_ranges", "mgs_segmented_variant_count", "mgs_segmented_variant",
+    "mgs_temp_status", "mgs_status_str", "mgs_last_hip_error", "mgs_version",
 ]
 
 MAX_KEYS = 0xFFFF0000  # MGS_MAX_KEYS
@@ -71,6 +71,10 @@ def _load() -> ctypes.CDLL:
     lib.mgs_fill_max.restype = i
     lib.mgs_segmented_sort_temp_bytes.argtypes = [i, i, sz, sz]
     lib.mgs_segmented_sort_temp_bytes.restype = sz
+    lib.mgs_segmented_variant_count.argtypes = []
+    lib.mgs_segmented_variant_count.restype = i
+    lib.mgs_segmented_variant.argtypes = [i, i, i, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    lib.mgs_segmented_variant.restype = i
     lib.mgs_segmented_sort_temp_bytes_tuned.argtypes = [i, i, sz, sz, vp]
     lib.mgs_segmented_sort_temp_bytes_tuned.restype = sz
     lib.mgs_temp_status.argtypes = [vp, sz, vp]

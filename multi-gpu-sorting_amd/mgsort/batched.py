@@ -9,7 +9,7 @@ so stability is observable.
 """
 from __future__ import annotations
 
-import math
+import ctypes
 import time
 from dataclasses import dataclass
 from typing import Dict, List, Optional, Sequence
@@ -17,7 +17,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 import torch
 
-from . import (KEY_U64, VAL_U32, Tuning, lib, segmented_sort, temp_buffer)
+from . import (KEY_U64, VAL_U32, lib, segmented_sort, temp_buffer, tuning)
 
 MASK64 = (1 << 64) - 1
 
@@ -131,43 +131,17 @@ class BatchedResult:
     batched_ms: float
     loop_ms: float
     loop_graph_ms: Optional[float]
-    tuning: Dict[str, int]
+    tuning: Dict[str, object]
     verified: bool
 
 
-def autotune(keys: torch.Tensor, vals: torch.Tensor, d_off: torch.Tensor, out_k, out_v, temp, stream,
-             candidates: Sequence[Dict[str, int]], reps: int = 2) -> Dict[str, int]:
-    """Times every candidate tuning on the workload itself; returns the fastest."""
-    best, best_t = None, math.inf
-    for c in candidates:
-        t = Tuning(c.get("large_variant", 0), c.get("small_max", 0), c.get("medium_max", 0), c.get("max_segment_keys", 0))
-        segmented_sort(KEY_U64, keys, d_off, vals, keys_out=out_k, vals_out=out_v, tuning=t, temp=temp,
-                       stream=stream)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            segmented_sort(KEY_U64, keys, d_off, vals, keys_out=out_k, vals_out=out_v, tuning=t, temp=temp,
-                           stream=stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        if ms < best_t:
-            best, best_t = c, ms
-    return dict(best)
-
-
-DEFAULT_CANDIDATES = [
-    {"small_max": 0, "medium_max": 0},
-    {"small_max": 0, "medium_max": 1 << 15},
-    {"small_max": 0, "medium_max": 1 << 16},
-    {"small_max": 0, "medium_max": 1 << 18},
-    {"small_max": 1024, "medium_max": 1 << 17},
-]
-
-
 def run(dist: str = "uniform", nseg: int = 4096, seed: int = 2024, reps: int = 3, loop_graph: bool = True,
-        candidates=DEFAULT_CANDIDATES, log=print) -> BatchedResult:
+        tune_now: bool = False, log=print) -> BatchedResult:
+    """The batch as one segmented sort (captured in a graph and replayed)
+    against one mgs_sort per array.  The segmented sort's tuning comes from
+    the persisted table (mgsort.tuning) for u64 keys + u32 payloads at this
+    batch's size, or, without a table entry (or with tune_now), from timing
+    every point of the search space on this batch."""
     dev = torch.device("cuda")
     lengths = segment_lengths(nseg, seed=seed)
     offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
@@ -178,11 +152,22 @@ def run(dist: str = "uniform", nseg: int = 4096, seed: int = 2024, reps: int = 3
     out_k, out_v = torch.empty_like(keys), torch.empty_like(vals)
     d_off = torch.from_numpy(offsets).to(dev)
     stream = torch.cuda.Stream()
-    need = lib.mgs_segmented_sort_temp_bytes(KEY_U64, VAL_U32, total, nseg)
+    found = None if tune_now else tuning.lookup(KEY_U64, VAL_U32, total)
+    if found is not None:
+        tn, entry = found
+        point = {"large_variant": tn.large_variant, "small_max": tn.small_max, "medium_max": tn.medium_max}
+        tuned = {"point": point, "source": f"table ({entry['key']}+{entry['val']}, 2^{entry['size_log2']} keys)",
+                 "table_ms": entry.get("ms"), "table_default_ms": entry.get("default_ms")}
+    else:
+        with torch.cuda.stream(stream):
+            point, best_ms, default_ms, timings = tuning.tune(KEY_U64, keys, d_off, vals, out_k, out_v, stream)
+        tn = tuning.as_tuning(point)
+        tuned = {"point": point, "source": f"timed on this batch ({len(timings)} points)",
+                 "ms": round(best_ms, 3), "default_ms": round(default_ms, 3)}
+    blk, ipt = tuning.geometry(KEY_U64, VAL_U32, tn.large_variant)
+    tuned["geometry"] = {"block": blk, "items_per_thread": ipt}
+    need = lib.mgs_segmented_sort_temp_bytes_tuned(KEY_U64, VAL_U32, total, nseg, ctypes.byref(tn))
     temp = temp_buffer(need, dev)
-    with torch.cuda.stream(stream):
-        tuned = autotune(keys, vals, d_off, out_k, out_v, temp, stream, candidates)
-    tn = Tuning(tuned.get("large_variant", 0), tuned.get("small_max", 0), tuned.get("medium_max", 0), tuned.get("max_segment_keys", 0))
 
     # the whole batch as one graph, replayed
     g = torch.cuda.CUDAGraph()

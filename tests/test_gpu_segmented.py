@@ -50,6 +50,32 @@ def test_segmented(mg, kt, vdt, case):
         assert np.array_equal(from_dev(dv, vdt), ev)
 
 
+@pytest.mark.parametrize("kt,vdt", [(O.K_U32, None), (O.K_I32, np.uint32), (O.K_U64, None), (O.K_U64, np.uint32),
+                                    (O.K_F64, np.uint64)], ids=["u32", "i32+v32", "u64", "u64+v32", "f64+v64"])
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_segmented_large_variants(mg, kt, vdt, variant):
+    """Every tile geometry of the large segments' passes (the autotuner's
+    search space) gives the oracle's result; segments from one key to a few
+    tiles of the largest geometry, all of them large (medium_max = 1)."""
+    import torch
+    rng = np.random.default_rng(100 + variant + 7 * kt)
+    lengths = np.concatenate([[0, 1, 257, 4095, 4096, 4097, 18431, 18432, 18433],
+                              np.round(2.0 ** (9 + 8 * rng.random(40))).astype(np.int64)])
+    total = int(lengths.sum())
+    keys = random_keys(kt, total, rng, "few" if variant == 1 else "uniform")
+    vals = None if vdt is None else rng.integers(0, 2**31, size=total).astype(vdt)
+    off = make_offsets(lengths)
+    ek, ev = O.segmented_sort(kt, keys, off, vals)
+    dk, dv = to_dev(keys), to_dev(vals)
+    doff = torch.from_numpy(off.view(np.int64)).cuda()
+    t = mg.Tuning(variant, 1, 1, 0)
+    mg.segmented_sort(KEY_NAME[kt], dk, doff, dv, tuning=t)
+    torch.cuda.synchronize()
+    assert np.array_equal(from_dev(dk, keys.dtype), ek)
+    if vdt is not None:
+        assert np.array_equal(from_dev(dv, vdt), ev)
+
+
 @pytest.mark.parametrize("small_max,medium_max", [(1, 1), (64, 1 << 14), (4096, 1 << 20)])
 def test_segmented_bins_agree(mg, small_max, medium_max):
     """Every bin assignment gives the same result (routing is invisible)."""
