@@ -433,7 +433,8 @@ def serving_section(requests=192):
 def cpu_section(H, np, keys, gpu_sorted):
     """The reference CPU path on the GPU box's host: PARADIS on the headline
     input (10 repetitions), the GNU parallel sort (one run on 2^28 keys) and
-    __gnu_parallel::multiway_merge of 8 sorted runs of 2^27 keys."""
+    __gnu_parallel::multiway_merge of 8 sorted runs of 2^27 keys, with the
+    device k-way merge of the same runs beside it."""
     threads = host_threads()
     n = keys.size
     ts, exact = [], True
@@ -477,6 +478,34 @@ def cpu_section(H, np, keys, gpu_sorted):
                                     "gkeys_s": round(n / statistics.median(tm) / 1e9, 4),
                                     "host_gbps": round(8 * n / statistics.median(tm) / 1e9, 2),
                                     "bit_exact": ok}
+    del dst
+    # the same 8 runs merged on the device (mgs_multiway_merge, the HET sort's
+    # device merge backend)
+    import torch
+    import mgsort
+    d_src = torch.from_numpy(src.view(np.int32)).cuda()
+    del src
+    d_out = torch.empty_like(d_src)
+    druns = [d_src[r * rl:(r + 1) * rl] for r in range(runs)]
+    temp = mgsort.temp_buffer(mgsort.lib.mgs_multiway_merge_temp_bytes(mgsort.KEY_U32, runs, n), "cuda")
+    gm = []
+    for rep in range(6):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        mgsort.multiway_merge("u32", druns, d_out, temp=temp)
+        e1.record()
+        torch.cuda.synchronize()
+        if rep > 0:
+            gm.append(e0.elapsed_time(e1))
+    gok = mgsort.temp_status(temp) == 0 and bool(np.array_equal(d_out.cpu().numpy().view(np.uint32), gpu_sorted))
+    ms = statistics.median(gm)
+    par["gpu_multiway_merge_8_runs"] = {"keys": n, "ms_median": round(ms, 3), "gkeys_s": round(n / ms / 1e6, 2),
+                                        "hbm_gbps_alg": round(8 * n / ms / 1e6, 1),
+                                        "roofline_frac": round(8 * n / ms / 1e6 / PEAK_GBPS, 4),
+                                        "speedup_vs_cpu": round(statistics.median(tm) * 1e3 / ms, 2),
+                                        "bit_exact": gok}
+    del d_src, d_out, druns, temp
+    torch.cuda.empty_cache()
     return {"cpu_baseline": base, "cpu_parallel": par}
 
 

@@ -64,8 +64,10 @@ enum {
   MGS_ERR_DEVICE = -6,         /* a kernel gave up a bounded wait or found a  */
                                /* look-back chain not fully served: the       */
                                /* output is not valid (mgs_temp_status)       */
-  MGS_ERR_SKIPPED = -7         /* segments left unsorted: malformed bounds,  */
+  MGS_ERR_SKIPPED = -7,        /* segments left unsorted: malformed bounds,  */
                                /* beyond max_total_keys or max_segment_keys   */
+                               /* (mgs_temp_status)                           */
+  MGS_ERR_UNSORTED = -8        /* mgs_multiway_merge: input runs not sorted   */
                                /* (mgs_temp_status)                           */
 };
 
@@ -166,6 +168,35 @@ int mgs_merge(mgs_key kt, mgs_val vt,
               const void* b_keys, const void* b_vals, size_t nb,
               void* out_keys, void* out_vals,
               void* temp, size_t temp_bytes, void* stream);
+
+/* --------------------------------------------------------------------------
+ * k-way merge.  The device counterpart of __gnu_parallel::multiway_merge of
+ * sorted runs at reference src/gowanlock_sort.cuh:91-92 (MergeChunks and
+ * MergeMergeGroups), for runs that sit in device memory.  runs and run_lens
+ * are HOST arrays of k <= MGS_MULTIWAY_MAX_RUNS device pointers and lengths;
+ * every run sorted in enc order.  Writes the stable merge (equal keys in run
+ * order, then index order) of all runs to out, which must not overlap them.
+ * Runs that are not sorted give an unspecified order and may set
+ * MGS_ERR_UNSORTED in the status word (mgs_temp_status).  Keys only.
+ * Scratch: mgs_multiway_merge_temp_bytes(kt, k, sum of run_lens).  The call
+ * copies the run table from host memory, so it is not for graph capture.
+ * ------------------------------------------------------------------------ */
+#define MGS_MULTIWAY_MAX_RUNS 256
+size_t mgs_multiway_merge_temp_bytes(mgs_key kt, size_t k, size_t total_keys);
+
+int mgs_multiway_merge(mgs_key kt, const void* const* runs, const size_t* run_lens, size_t k,
+                       void* out, void* temp, size_t temp_bytes, void* stream);
+
+/* The same merge in `portions` consecutive stretches of the output, for
+ * callers that move the output on while the rest merges (the heterogeneous
+ * sort's copy back to the host).  Waits on the host once, after the
+ * splitters, for the stretch boundaries: portion_end[p] is the output index
+ * where stretch p ends, and portion_done[p] (an event of the caller,
+ * hipEvent_t, or portion_done NULL) is recorded on `stream` once stretch p is
+ * written. */
+int mgs_multiway_merge_portions(mgs_key kt, const void* const* runs, const size_t* run_lens, size_t k,
+                                void* out, void* temp, size_t temp_bytes, void* stream,
+                                size_t portions, size_t* portion_end, void* const* portion_done);
 
 /* --------------------------------------------------------------------------
  * Tanasic pivot.  Replaces SelectPivot/FindPivot at reference

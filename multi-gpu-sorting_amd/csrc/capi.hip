@@ -100,6 +100,45 @@ int mgs_swap_ranges(mgs_key kt, mgs_val vt, void* a_keys, void* a_vals, void* b_
   return mgs::swap_dispatch(kt, vt, a_keys, a_vals, b_keys, b_vals, n, as_stream(stream));
 }
 
+size_t mgs_multiway_merge_temp_bytes(mgs_key kt, size_t k, size_t total_keys) {
+  return mgs::multiway_temp_bytes(kt, k, total_keys);
+}
+
+static int multiway_checks(mgs_key kt, const void* const* runs, const size_t* run_lens, size_t k, const void* out,
+                           size_t* total) {
+  if (key_bytes(kt) < 0) return MGS_ERR_BAD_TYPE;
+  if (k > MGS_MULTIWAY_MAX_RUNS || (k > 0 && (runs == nullptr || run_lens == nullptr))) return MGS_ERR_INVALID;
+  size_t n = 0;
+  for (size_t j = 0; j < k; ++j) {
+    if (run_lens[j] > 0 && runs[j] == nullptr) return MGS_ERR_INVALID;
+    if (run_lens[j] > kMaxKeys || n + run_lens[j] < n) return MGS_ERR_TOO_LARGE;
+    n += run_lens[j];
+  }
+  if (n > 0 && out == nullptr) return MGS_ERR_INVALID;
+  *total = n;
+  return MGS_OK;
+}
+
+int mgs_multiway_merge(mgs_key kt, const void* const* runs, const size_t* run_lens, size_t k, void* out, void* temp,
+                       size_t temp_bytes, void* stream) {
+  size_t n = 0;
+  const int rc = multiway_checks(kt, runs, run_lens, k, out, &n);
+  if (rc != MGS_OK) return rc;
+  if (n == 0) return MGS_OK;
+  return mgs::multiway_dispatch(kt, runs, run_lens, k, out, temp, temp_bytes, as_stream(stream), 0, nullptr, nullptr);
+}
+
+int mgs_multiway_merge_portions(mgs_key kt, const void* const* runs, const size_t* run_lens, size_t k, void* out,
+                                void* temp, size_t temp_bytes, void* stream, size_t portions, size_t* portion_end,
+                                void* const* portion_done) {
+  size_t n = 0;
+  const int rc = multiway_checks(kt, runs, run_lens, k, out, &n);
+  if (rc != MGS_OK) return rc;
+  if (portions == 0 || portion_end == nullptr) return MGS_ERR_INVALID;
+  return mgs::multiway_dispatch(kt, runs, run_lens, k, out, temp, temp_bytes, as_stream(stream), portions, portion_end,
+                                reinterpret_cast<hipEvent_t*>(const_cast<void**>(portion_done)));
+}
+
 int mgs_fill_max(mgs_key kt, void* keys, size_t n, void* stream) {
   if (key_bytes(kt) < 0) return MGS_ERR_BAD_TYPE;
   return mgs::fill_max_dispatch(kt, keys, n, as_stream(stream));
@@ -152,6 +191,7 @@ int mgs_temp_status(const void* temp, size_t temp_bytes, void* stream) {
   if (e != hipSuccess) return mgs::record_hip(e);
   if (word & (mgs::kStatusLookback | mgs::kStatusChainUnserved)) return MGS_ERR_DEVICE;
   if (word & mgs::kStatusSkipped) return MGS_ERR_SKIPPED;
+  if (word & mgs::kStatusUnsorted) return MGS_ERR_UNSORTED;
   return MGS_OK;
 }
 
@@ -165,6 +205,7 @@ const char* mgs_status_str(int status) {
     case MGS_ERR_TOO_LARGE: return "too many keys";
     case MGS_ERR_DEVICE: return "a kernel detected an invalid look-back state; output not valid";
     case MGS_ERR_SKIPPED: return "segments left unsorted (malformed or out-of-bound segments)";
+    case MGS_ERR_UNSORTED: return "merge input runs not sorted; output not valid";
   }
   return "unknown status";
 }

@@ -70,6 +70,10 @@ const char* last_stage_names();
 size_t merge_temp_bytes(mgs_key kt, mgs_val vt, size_t na, size_t nb);
 int merge_dispatch(mgs_key kt, mgs_val vt, const void* ak, const void* av, size_t na, const void* bk, const void* bv,
                    size_t nb, void* ok, void* ov, void* temp, size_t temp_bytes, hipStream_t s);
+size_t multiway_temp_bytes(mgs_key kt, size_t k, size_t total);
+int multiway_dispatch(mgs_key kt, const void* const* runs, const size_t* lens, size_t k, void* out, void* temp,
+                      size_t temp_bytes, hipStream_t s, size_t portions, size_t* portion_end,
+                      hipEvent_t* portion_done);
 int pivot_dispatch(mgs_key kt, const void* const* left, const void* const* right, size_t part_size, size_t num_parts,
                    uint64_t* result, hipStream_t s);
 int swap_dispatch(mgs_key kt, mgs_val vt, void* ak, void* av, void* bk, void* bv, size_t n, hipStream_t s);

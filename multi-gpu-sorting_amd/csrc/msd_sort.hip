@@ -355,7 +355,7 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
   __shared__ uint32_t gbase[kRadix];
   __shared__ uint32_t s_prefix[kRadix];
   __shared__ uint32_t sums[4];
-  __shared__ uint32_t s_tile, s_chain;
+  __shared__ uint32_t s_tile;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);

@@ -26,6 +26,7 @@ enum : uint32_t {
   kStatusLookback = 1u,        // a look-back wait reached its bound
   kStatusChainUnserved = 2u,   // a chain's tiles were not all taken
   kStatusSkipped = 4u,         // segments left unsorted (malformed bounds)
+  kStatusUnsorted = 8u,        // multiway merge: runs not sorted (a piece over its bound)
 };
 
 // Buffer ids of the ping-pong plan.

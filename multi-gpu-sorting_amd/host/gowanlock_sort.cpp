@@ -49,6 +49,9 @@ void SetupGowanlockSort(const std::vector<int>& gpus, int num_threads) {
 
 namespace {
 
+// Stretches of the device merge's output copied back one after another.
+constexpr size_t kMergePortions = 16;
+
 // Sorted-run counter shared with the merge thread (SyncState of the
 // reference, src/gowanlock_sort.cuh:20-27).
 struct Progress {
@@ -63,41 +66,6 @@ using Run = std::pair<T*, T*>;
 template <typename T>
 void MultiwayMerge(std::vector<Run<T>>& runs, T* out, size_t len) {
   __gnu_parallel::multiway_merge(runs.begin(), runs.end(), out, (std::ptrdiff_t)len, EncodedLess<T>());
-}
-
-// Pairwise merge-path rounds over sorted runs resident in `a` (scratch `b`);
-// returns the buffer holding the merged result.
-template <typename T>
-T* MergeRunsOnDevice(std::vector<std::pair<size_t, size_t>> runs, T* a, T* b, hipStream_t s) {
-  const mgs_key kt = KeyOf<T>::value;
-  size_t longest = 0;
-  for (auto& r : runs) longest = std::max(longest, r.second);
-  DeviceBuffer temp;
-  while (runs.size() > 1) {
-    std::vector<std::pair<size_t, size_t>> next;
-    for (size_t i = 0; i < runs.size(); i += 2) {
-      if (i + 1 == runs.size()) {
-        HIP_CHECK(hipMemcpyAsync(b + runs[i].first, a + runs[i].first, runs[i].second * sizeof(T),
-                                 hipMemcpyDeviceToDevice, s));
-        next.push_back(runs[i]);
-        continue;
-      }
-      const auto& x = runs[i];
-      const auto& y = runs[i + 1];
-      const size_t need = mgs_merge_temp_bytes(kt, MGS_VAL_NONE, x.second, y.second);
-      if (temp.bytes() < need) {
-        HIP_CHECK(hipStreamSynchronize(s));
-        temp.Allocate(need);
-      }
-      MGS_CHECK(mgs_merge(kt, MGS_VAL_NONE, a + x.first, nullptr, x.second, a + y.first, nullptr, y.second,
-                          b + x.first, nullptr, temp.get(), temp.bytes(), s));
-      next.push_back({x.first, x.second + y.second});
-    }
-    runs.swap(next);
-    std::swap(a, b);
-  }
-  HIP_CHECK(hipStreamSynchronize(s));
-  return a;
 }
 
 }  // namespace
@@ -133,11 +101,13 @@ void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_grou
   const mgs_key kt = KeyOf<T>::value;
   auto chunk_len = [&](size_t c) { return std::min(cs, n - c * cs); };
 
-  bool gpu_merge = merge_backend == "gpu" && C > 1;
+  bool gpu_merge = merge_backend == "gpu" && C > 1 && C <= MGS_MULTIWAY_MAX_RUNS;
+  const size_t merge_temp_bytes = gpu_merge ? mgs_multiway_merge_temp_bytes(kt, C, n) : 0;
   if (gpu_merge) {
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-    const size_t need = 2 * n * sizeof(T) + mgs_sort_temp_bytes(kt, MGS_VAL_NONE, cs) + (size_t(1) << 28);
+    const size_t need = 2 * n * sizeof(T) + std::max(mgs_sort_temp_bytes(kt, MGS_VAL_NONE, cs), merge_temp_bytes) +
+                        (size_t(1) << 28);
     gpu_merge = need < free_b;
   }
   const size_t temp_bytes = mgs_sort_temp_bytes(kt, MGS_VAL_NONE, cs);
@@ -156,9 +126,10 @@ void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_grou
   std::vector<DeviceBuffer> bufs(gpu_merge ? 0 : 2);
   DeviceBuffer resident, resident_alt, sort_temp;
   if (gpu_merge) {
+    // the sorts' scratch serves the merge afterwards
     resident.Allocate(n * sizeof(T));
     resident_alt.Allocate(n * sizeof(T));
-    sort_temp.Allocate(temp_bytes);
+    sort_temp.Allocate(std::max(temp_bytes, merge_temp_bytes));
   } else if (num_buffers == 2) {
     for (auto& b : bufs) b.Allocate(std::max(cs * sizeof(T), temp_bytes));
   } else {
@@ -295,11 +266,34 @@ void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_grou
   if (gpu_merge) {
     comp.Sync();
     TimeDurations::Get()->Tic("merge_phase");
-    std::vector<std::pair<size_t, size_t>> runs;
-    for (size_t c = 0; c < C; ++c) runs.push_back({c * cs, chunk_len(c)});
-    T* res = MergeRunsOnDevice(runs, resident.get<T>(), resident_alt.get<T>(), comp.get());
-    HIP_CHECK(hipMemcpyAsync(primary, res, n * sizeof(T), hipMemcpyDeviceToHost, comp));
-    comp.Sync();
+    // one k-way merge of the resident runs (mgs_multiway_merge), written in
+    // stretches that go back to the host while the next ones merge
+    std::vector<const void*> runs(C);
+    std::vector<size_t> lens(C);
+    for (size_t c = 0; c < C; ++c) {
+      runs[c] = resident.get<T>() + c * cs;
+      lens[c] = chunk_len(c);
+    }
+    const size_t portions = std::min<size_t>(kMergePortions, std::max<size_t>(1, n / (size_t(1) << 20)));
+    std::vector<size_t> ends(portions);
+    std::vector<std::unique_ptr<Event>> done;
+    std::vector<void*> evs;
+    for (size_t p = 0; p < portions; ++p) {
+      done.emplace_back(new Event());
+      evs.push_back(done.back()->get());
+    }
+    T* merged = resident_alt.get<T>();
+    MGS_CHECK(mgs_multiway_merge_portions(kt, runs.data(), lens.data(), C, merged, sort_temp.get(), sort_temp.bytes(),
+                                          comp, portions, ends.data(), evs.data()));
+    size_t at = 0;
+    for (size_t p = 0; p < portions; ++p) {
+      HIP_CHECK(hipStreamWaitEvent(d2h, *done[p], 0));
+      if (ends[p] > at)
+        HIP_CHECK(hipMemcpyAsync(primary + at, merged + at, (ends[p] - at) * sizeof(T), hipMemcpyDeviceToHost, d2h));
+      at = ends[p];
+    }
+    d2h.Sync();
+    MGS_CHECK(mgs_temp_status(sort_temp.get(), sort_temp.bytes(), comp));
     TimeDurations::Get()->Toc("merge_phase");
   } else {
     for (size_t c = 0; c < C; ++c) {

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence
 
 import torch
 
@@ -32,6 +32,7 @@ SYMBOLS = [
     "mgs_sort_temp_bytes", "mgs_sort", "mgs_sort_noalloc", "mgs_sort_timed", "mgs_sort_timed_stages", "mgs_sort_path", "mgs_merge_temp_bytes", "mgs_merge", "mgs_select_pivot",
     "mgs_swap_ranges", "mgs_fill_max", "mgs_segmented_sort_temp_bytes", "mgs_segmented_sort_temp_bytes_tuned",
     "mgs_segmented_sort", "mgs_segmented_sort_ranges", "mgs_segmented_variant_count", "mgs_segmented_variant",
+    "mgs_multiway_merge_temp_bytes", "mgs_multiway_merge", "mgs_multiway_merge_portions",
     "mgs_temp_status", "mgs_status_str", "mgs_last_hip_error", "mgs_version",
 ]
 
@@ -71,6 +72,13 @@ def _load() -> ctypes.CDLL:
     lib.mgs_fill_max.restype = i
     lib.mgs_segmented_sort_temp_bytes.argtypes = [i, i, sz, sz]
     lib.mgs_segmented_sort_temp_bytes.restype = sz
+    lib.mgs_multiway_merge_temp_bytes.argtypes = [i, sz, sz]
+    lib.mgs_multiway_merge_temp_bytes.restype = sz
+    lib.mgs_multiway_merge.argtypes = [i, ctypes.POINTER(vp), ctypes.POINTER(sz), sz, vp, vp, sz, vp]
+    lib.mgs_multiway_merge.restype = i
+    lib.mgs_multiway_merge_portions.argtypes = [i, ctypes.POINTER(vp), ctypes.POINTER(sz), sz, vp, vp, sz, vp, sz,
+                                                ctypes.POINTER(sz), ctypes.POINTER(vp)]
+    lib.mgs_multiway_merge_portions.restype = i
     lib.mgs_segmented_variant_count.argtypes = []
     lib.mgs_segmented_variant_count.restype = i
     lib.mgs_segmented_variant.argtypes = [i, i, i, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
@@ -173,6 +181,30 @@ def merge(kt, a: torch.Tensor, b: torch.Tensor, out: torch.Tensor, av=None, bv=N
     st = lib.mgs_merge(kt, vt, _ptr(a), _ptr(av), a.numel(), _ptr(b), _ptr(bv), b.numel(), _ptr(out),
                        _ptr(out_vals), _ptr(temp), temp.numel(), _stream(stream))
     _check(st, "mgs_merge")
+
+
+def multiway_merge(kt, runs: Sequence[torch.Tensor], out: torch.Tensor, temp=None, stream=None,
+                   portions: int = 0) -> Optional[List[int]]:
+    """Stable k-way merge of sorted device runs into out (keys only).  With
+    portions > 0 the output is written in that many stretches; returns their
+    end indices."""
+    kt = key_type(kt)
+    k = len(runs)
+    ptrs = (ctypes.c_void_p * max(k, 1))(*[r.data_ptr() for r in runs])
+    lens = (ctypes.c_size_t * max(k, 1))(*[r.numel() for r in runs])
+    total = sum(r.numel() for r in runs)
+    need = lib.mgs_multiway_merge_temp_bytes(kt, k, total)
+    if temp is None:
+        temp = temp_buffer(need, out.device, stream)
+    if portions <= 0:
+        st = lib.mgs_multiway_merge(kt, ptrs, lens, k, _ptr(out), _ptr(temp), temp.numel(), _stream(stream))
+        _check(st, "mgs_multiway_merge")
+        return None
+    ends = (ctypes.c_size_t * portions)()
+    st = lib.mgs_multiway_merge_portions(kt, ptrs, lens, k, _ptr(out), _ptr(temp), temp.numel(), _stream(stream),
+                                         portions, ends, None)
+    _check(st, "mgs_multiway_merge_portions")
+    return list(ends)
 
 
 def select_pivot(kt, left: Sequence[torch.Tensor], right: Sequence[torch.Tensor], part_size: int,

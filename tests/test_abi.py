@@ -171,3 +171,26 @@ def test_cpu_baselines_sort(dtype, np_t):
             assert fn(dtype.encode(), k.ctypes.data, n, threads) == 0
             # equal keys are indistinguishable in a keys-only sort, so compare encodings
             assert np.array_equal(O.encode(kt, k.view(O.BITS_DTYPE[kt])), O.encode(kt, exp))
+
+
+def test_multiway_merge_argument_checks():
+    """Rejected before any device work: unknown type, too many runs, NULL run
+    with keys, NULL output; no runs at all is a no-op."""
+    lib = ctypes.CDLL(LIB)
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    lib.mgs_multiway_merge.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(sz), sz, vp, vp, sz, vp]
+    lib.mgs_multiway_merge_temp_bytes.argtypes = [ctypes.c_int, sz, sz]
+    lib.mgs_multiway_merge_temp_bytes.restype = sz
+    runs = (vp * 2)(16, None)
+    lens = (sz * 2)(10, 5)
+    assert lib.mgs_multiway_merge(9, runs, lens, 2, 16, 16, 1 << 20, None) == -1
+    assert lib.mgs_multiway_merge(0, runs, lens, 257, 16, 16, 1 << 20, None) == -4
+    assert lib.mgs_multiway_merge(0, runs, lens, 2, 16, 16, 1 << 20, None) == -4   # run 1 NULL with 5 keys
+    lens2 = (sz * 2)(10, 0)
+    assert lib.mgs_multiway_merge(0, runs, lens2, 2, None, 16, 1 << 20, None) == -4  # no output
+    assert lib.mgs_multiway_merge(0, None, None, 0, None, None, 0, None) == 0
+    # scratch grows with the keys and the run count, and 0 for too many runs
+    a = lib.mgs_multiway_merge_temp_bytes(0, 8, 1 << 30)
+    assert a > lib.mgs_multiway_merge_temp_bytes(0, 8, 1 << 20) > 0
+    assert lib.mgs_multiway_merge_temp_bytes(0, 257, 1 << 20) == 0
+    assert a < (1 << 30) * 4 // 8  # well below the keys themselves

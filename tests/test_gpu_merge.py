@@ -148,3 +148,69 @@ def test_swap(mg, n):
     torch.cuda.synchronize()
     assert np.array_equal(from_dev(da, np.uint64), b)
     assert np.array_equal(from_dev(db, np.uint64), a)
+
+
+# ---------------------------------------------------------------------------
+# k-way merge (mgs_multiway_merge): the device counterpart of
+# __gnu_parallel::multiway_merge at src/gowanlock_sort.cuh:91-92.  Expected
+# output: the stable sort of the runs laid end to end (equal keys in run
+# order), from the oracle's stable sort.
+# ---------------------------------------------------------------------------
+def kway_lengths(k, rng, scale):
+    lens = rng.integers(0, scale, size=k)
+    if k >= 3:
+        lens[1] = 0  # an empty run
+        lens[-1] = scale * 3  # one long run
+    return lens
+
+
+@pytest.mark.parametrize("kt", [O.K_U32, O.K_U64, O.K_I32, O.K_F32, O.K_F64], ids=lambda k: KEY_NAME[k])
+@pytest.mark.parametrize("k", [1, 2, 3, 8, 17, 64, 256])
+@pytest.mark.parametrize("dist", ["uniform", "few", "equal"])
+def test_multiway_merge(mg, kt, k, dist):
+    import torch
+    rng = np.random.default_rng(1000 * k + 7 * kt + len(dist))
+    scale = {1: 300_000, 2: 200_000, 3: 100_000, 8: 60_000, 17: 20_000, 64: 5000, 256: 1500}[k]
+    lens = kway_lengths(k, rng, scale)
+    runs = [sorted_keys(kt, int(n), rng, dist) for n in lens]
+    expect = O.sort(kt, np.concatenate(runs) if runs else np.zeros(0, dtype=np.uint32))[0]
+    druns = [to_dev(r) for r in runs]
+    out = torch.empty(int(lens.sum()), dtype=druns[0].dtype, device="cuda")
+    temp = mg.temp_buffer(mg.lib.mgs_multiway_merge_temp_bytes(kt, k, int(lens.sum())), "cuda")
+    mg.multiway_merge(KEY_NAME[kt], druns, out, temp=temp)
+    assert mg.temp_status(temp) == 0
+    assert np.array_equal(from_dev(out, expect.dtype), expect)
+
+
+@pytest.mark.parametrize("portions", [1, 3, 16])
+def test_multiway_merge_portions(mg, portions):
+    """Stretches: ends increase to the total, and the output equals the
+    one-call merge."""
+    import torch
+    rng = np.random.default_rng(portions)
+    runs = [sorted_keys(O.K_U64, n, rng) for n in (1 << 18, 77_777, 0, 300_001, 5, 1 << 17, 4096, 65_537)]
+    total = sum(r.size for r in runs)
+    expect = O.sort(O.K_U64, np.concatenate(runs))[0]
+    druns = [to_dev(r) for r in runs]
+    out = torch.empty(total, dtype=torch.int64, device="cuda")
+    ends = mg.multiway_merge("u64", druns, out, portions=portions)
+    torch.cuda.synchronize()
+    assert len(ends) == portions and ends[-1] == total
+    assert all(a <= b for a, b in zip(ends, ends[1:]))
+    assert np.array_equal(from_dev(out, np.uint64), expect)
+
+
+def test_multiway_merge_unsorted_runs_report(mg):
+    """Runs that are not sorted: every access stays inside the runs and the
+    output; the call either reports MGS_ERR_UNSORTED or writes some order."""
+    import torch
+    rng = np.random.default_rng(5)
+    runs = [random_keys(O.K_U32, n, rng) for n in (100_000, 50_000, 120_000)]
+    runs[1] = np.sort(runs[1])[::-1].copy()
+    druns = [to_dev(r) for r in runs]
+    total = sum(r.size for r in runs)
+    out = torch.zeros(total + 4096, dtype=torch.int32, device="cuda")
+    temp = mg.temp_buffer(mg.lib.mgs_multiway_merge_temp_bytes(O.K_U32, 3, total), "cuda")
+    mg.multiway_merge("u32", druns, out[:total], temp=temp)
+    assert mg.temp_status(temp) in (0, -8)
+    assert int(out[total:].abs().sum().item()) == 0  # nothing written past the output
