@@ -12,6 +12,7 @@
 #include <thread>
 #include <utility>
 
+#include "chunk_merger.h"
 #include "device_buffer.h"
 #include "hip_check.h"
 #include "key_traits.h"
@@ -52,41 +53,10 @@ namespace {
 // Stretches of the device merge's output copied back one after another.
 constexpr size_t kMergePortions = 16;
 
-// Sorted-run counter shared with the merge thread (SyncState of the
-// reference, src/gowanlock_sort.cuh:20-27).
-struct Progress {
-  std::mutex mu;
-  std::condition_variable cv;
-  size_t sorted = 0;
-};
-
 template <typename T>
 using Run = std::pair<T*, T*>;
 
-template <typename T>
-void MultiwayMerge(std::vector<Run<T>>& runs, T* out, size_t len) {
-  __gnu_parallel::multiway_merge(runs.begin(), runs.end(), out, (std::ptrdiff_t)len, EncodedLess<T>());
-}
-
 }  // namespace
-
-// Keeps the merge thread from outliving an exception: wakes it with every
-// run marked sorted and joins it (its output is discarded with the error).
-struct MergerGuard {
-  std::thread* t;
-  Progress* prog;
-  size_t all;
-  ~MergerGuard() {
-    if (t->joinable()) {
-      {
-        std::lock_guard<std::mutex> lk(prog->mu);
-        prog->sorted = all;
-      }
-      prog->cv.notify_all();
-      t->join();
-    }
-  }
-};
 
 template <typename T>
 void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_group_size, size_t num_buffers,
@@ -159,29 +129,10 @@ void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_grou
   auto run_dst = [&](size_t c) {
     return (c < plan.num_chunks_to_merge || plan.final_runs == 1) ? primary + c * cs : secondary.data() + c * cs;
   };
-  Progress prog;
-  std::thread merger;
-  MergerGuard guard{&merger, &prog, C};
-  if (!gpu_merge && plan.num_merge_groups_to_merge > 0) {
-    // MergeChunks: merge each group of mgs sorted runs into `secondary` as
-    // soon as its last run has landed
-    merger = std::thread([&] {
-      for (size_t gidx = 0; gidx < plan.num_merge_groups_to_merge; ++gidx) {
-        {
-          std::unique_lock<std::mutex> lk(prog.mu);
-          prog.cv.wait(lk, [&] { return prog.sorted >= (gidx + 1) * mgs; });
-        }
-        std::vector<Run<T>> runs;
-        size_t len = 0;
-        const size_t first = gidx * mgs;
-        for (size_t c = first; c < first + mgs; ++c) {
-          runs.push_back({primary + c * cs, primary + c * cs + chunk_len(c)});
-          len += chunk_len(c);
-        }
-        MultiwayMerge(runs, secondary.data() + first * cs, len);
-      }
-    });
-  }
+  // MergeChunks: merges each group of mgs sorted runs into `secondary` as
+  // soon as its last run has landed (joined, on an exception, by its
+  // destructor)
+  ChunkMerger<T> merger(primary, secondary.data(), cs, n, gpu_merge ? 0 : plan.num_merge_groups_to_merge, mgs);
 
   if (gpu_merge) {
     for (size_t c = 0; c < C; ++c) {
@@ -298,13 +249,9 @@ void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_grou
   } else {
     for (size_t c = 0; c < C; ++c) {
       d2h_done[c]->Sync();
-      {
-        std::lock_guard<std::mutex> lk(prog.mu);
-        prog.sorted = c + 1;
-      }
-      prog.cv.notify_one();
+      merger.Landed(c + 1);
     }
-    if (merger.joinable()) merger.join();
+    merger.Join();
     TimeDurations::Get()->Tic("merge_phase");
     if (plan.final_runs > 1) {
       // MergeMergeGroups: the pre-merged groups and the remaining runs, all
