@@ -6,23 +6,22 @@
 // Stable order: the output is the stable sort of the runs laid end to end in
 // run order, i.e. the lexicographic order of (enc(key), run, index).
 //
-//   sample    every S-th key of every run (S = kCap / (kStep + k)) with its
-//             sample id, ids numbered run by run
+//   sample    every S-th key of every run (S = kCap / (step + k), step = 3k
+//             but at least 8) with its sample id, ids numbered run by run
 //   sort      the samples by key, stably (mgs_sort with the ids as payload):
 //             (key, run, index) order
-//   split     every kStep-th sorted sample (v, j, i) bounds a piece: run j'
+//   split     every step-th sorted sample (v, j, i) bounds a piece: run j'
 //             contributes the keys that precede (v, j, i) -- upper bound of v
 //             for j' < j, lower bound for j' > j, exactly i for j' = j
 //   base      each boundary's output position (the sum of its splits)
 //   piece     one workgroup per piece: its slices of the k runs into LDS,
-//             every key ranked by binary searches in the other slices
-//             (stable: equal keys of earlier runs count as smaller), and the
-//             piece written out with consecutive lanes on consecutive
-//             addresses
+//             merged by a tree of LDS merge-path rounds (adjacent slices,
+//             earlier runs first on ties), and the piece written out with
+//             consecutive lanes on consecutive addresses
 //
 // A run contributes to a piece only the keys between two of its consecutive
 // samples per sample of it inside the piece, plus one more stretch, so a
-// piece holds at most (kStep + k) S <= kCap keys and fits the LDS.
+// piece holds at most (step + k) S <= kCap keys and fits the LDS.
 #include <vector>
 
 #include "common.hpp"
@@ -35,10 +34,12 @@ namespace {
 
 constexpr int kMwBlock = 512;
 constexpr uint32_t kMwCap = 8192;  // keys per piece at most
-constexpr uint32_t kMwStep = 8;    // sorted samples per piece
+// sorted samples per piece: 3k (at least 8), so that a piece holds on
+// average step / (step + k) = 3/4 of the keys its LDS bound allows
+inline uint32_t piece_step(size_t k) { return (uint32_t)std::max<size_t>(8, 3 * k); }
 constexpr size_t kMwMaxRuns = 256;
 
-inline uint32_t sample_stride(size_t k) { return (uint32_t)(kMwCap / (kMwStep + k)); }
+inline uint32_t sample_stride(size_t k) { return (uint32_t)(kMwCap / (piece_step(k) + k)); }
 
 size_t align_up(size_t x) { return (x + 255) / 256 * 256; }
 
@@ -51,7 +52,7 @@ struct MwLayout {
 MwLayout mw_layout(int kb, size_t k, uint64_t ns) {
   MwLayout L{};
   L.ns = ns;
-  L.pieces = ns == 0 ? 0 : (ns - 1) / kMwStep + 1;
+  L.pieces = ns == 0 ? 0 : (ns - 1) / piece_step(k) + 1;
   size_t o = 0;
   auto take = [&](size_t bytes) {
     const size_t at = o;
@@ -121,7 +122,7 @@ template <int KT, typename Bits>
 __global__ void mw_split_kernel(const Bits* const* __restrict__ runs, const uint64_t* __restrict__ lens,
                                 const uint64_t* __restrict__ sbase, uint32_t k, uint32_t S,
                                 const Bits* __restrict__ skeys, const uint32_t* __restrict__ sid, uint64_t pieces,
-                                uint64_t* __restrict__ splits) {
+                                uint32_t step, uint64_t* __restrict__ splits) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (pieces + 1) * k) return;
   const uint64_t b = t / k;
@@ -134,7 +135,7 @@ __global__ void mw_split_kernel(const Bits* const* __restrict__ runs, const uint
     splits[t] = lens[jj];
     return;
   }
-  const uint64_t q = b * kMwStep;
+  const uint64_t q = b * step;
   const Bits e = Codec<KT>::enc(skeys[q]);
   const uint32_t id = sid[q];
   uint32_t lo = 0, hi = k;  // run of sample id
@@ -167,14 +168,15 @@ __global__ void mw_base_kernel(const uint64_t* __restrict__ splits, uint32_t k, 
   bases[b] = s;
 }
 
-// Keys of the LDS slice [p, p + n) that precede e (see bound_global).
-template <int KT, bool UPPER, typename Bits>
-__device__ __forceinline__ uint32_t bound_lds(const Bits* p, uint32_t n, Bits e) {
-  uint32_t lo = 0, hi = n;
+// Keys of `a` taken among the first `diag` outputs of the stable merge of
+// a (na keys, taken first on ties) and b (nb keys), both in LDS.
+template <int KT, typename Bits>
+__device__ __forceinline__ uint32_t corank_lds(const Bits* a, uint32_t na, const Bits* b, uint32_t nb, uint32_t diag) {
+  uint32_t lo = diag > nb ? diag - nb : 0;
+  uint32_t hi = diag < na ? diag : na;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
-    const Bits m = Codec<KT>::enc(p[mid]);
-    if (UPPER ? (m <= e) : (m < e))
+    if (Codec<KT>::enc(a[mid]) <= Codec<KT>::enc(b[diag - mid - 1]))
       lo = mid + 1;
     else
       hi = mid;
@@ -182,66 +184,129 @@ __device__ __forceinline__ uint32_t bound_lds(const Bits* p, uint32_t n, Bits e)
   return lo;
 }
 
+// One piece: its slices of the k runs are loaded into LDS back to back (run
+// order), then merged in ceil(log2 k) rounds, each merging adjacent slice
+// pairs (2i, 2i+1) into the other LDS buffer (earlier runs first on ties, so
+// the order stays stable).  In a round every thread writes kSlots
+// consecutive outputs: one co-rank search in its pair, then a serial merge.
 template <int KT, typename Bits>
 __global__ __launch_bounds__(kMwBlock) void mw_piece_kernel(const Bits* const* __restrict__ runs, uint32_t k,
                                                             const uint64_t* __restrict__ splits,
                                                             const uint64_t* __restrict__ bases, uint64_t first_piece,
                                                             Bits* __restrict__ out, uint32_t* __restrict__ status) {
-  __shared__ Bits in[kMwCap];
-  __shared__ Bits res[kMwCap];
-  __shared__ uint32_t off[kMwMaxRuns + 1];
-  __shared__ uint32_t s_bad;
+  // outputs per thread in a merge round: odd, so that the lanes' serial
+  // merges start in different LDS banks; the loads use kLoads per thread
+  constexpr uint32_t kSlots = kMwCap / kMwBlock + 1;
+  constexpr uint32_t kLoads = kMwCap / kMwBlock;
+  __shared__ Bits buf[2][kMwCap];
+  __shared__ uint32_t bnd[kMwMaxRuns + 1];
+  __shared__ uint64_t sbeg[kMwMaxRuns];
+  __shared__ const Bits* sptr[kMwMaxRuns];
+  __shared__ uint32_t sums[4];
   const int tid = threadIdx.x;
   const uint64_t b = first_piece + blockIdx.x;
-  const uint64_t* s0 = splits + b * k;
-  const uint64_t* s1 = s0 + k;
-  if (tid == 0) {
-    uint64_t acc = 0;
-    bool bad = false;
-    for (uint32_t j = 0; j < k; ++j) {
-      off[j] = (uint32_t)std::min<uint64_t>(acc, kMwCap);
-      bad = bad || s1[j] < s0[j];
-      acc += s1[j] > s0[j] ? s1[j] - s0[j] : 0;
-    }
-    // splits that go backwards or more than kCap keys (only unsorted runs
-    // break the sample bound): the piece is not merged and the call reports
-    // it; otherwise the piece's keys are exactly [bases[b], bases[b+1]) of
-    // the output
-    s_bad = (bad || acc > kMwCap) ? 1u : 0u;
-    off[k] = (uint32_t)std::min<uint64_t>(acc, kMwCap);
+  // slice j of the piece: runs[j][splits[b][j], splits[b+1][j]); slices that
+  // go backwards or more than kCap keys in all (only unsorted runs break the
+  // sample bound) leave the piece unmerged and the call reports it;
+  // otherwise the piece's keys are exactly [bases[b], bases[b+1]) of the
+  // output
+  uint32_t len = 0;
+  bool bad = false;
+  if ((uint32_t)tid < k) {
+    const uint64_t lo = splits[b * k + tid], hi = splits[(b + 1) * k + tid];
+    bad = hi < lo;
+    len = bad ? 0u : (uint32_t)std::min<uint64_t>(hi - lo, kMwCap + 1);
+    sbeg[tid] = lo;
+    sptr[tid] = runs[tid];
   }
-  __syncthreads();
-  if (s_bad) {
+  const uint32_t excl = block_excl_scan_256((uint32_t)tid < k ? len : 0u, sums, tid);
+  if ((uint32_t)tid < k) {
+    bnd[tid] = min(excl, kMwCap);
+    if ((uint32_t)tid == k - 1) {
+      bad = bad || excl + len > kMwCap;
+      bnd[k] = min(excl + len, kMwCap);
+    }
+  }
+  if (__syncthreads_or(bad ? 1 : 0)) {
     if (tid == 0) atomicOr(status, kStatusUnsorted);
     return;
   }
-  const uint32_t cnt = off[k];
-  for (uint32_t j = 0; j < k; ++j) {
-    const Bits* src = runs[j] + s0[j];
-    const uint32_t o = off[j], m = off[j + 1] - o;
-    for (uint32_t i = tid; i < m; i += kMwBlock) in[o + i] = src[i];
-  }
-  __syncthreads();
-  for (uint32_t e = tid; e < cnt; e += kMwBlock) {
-    uint32_t lo = 0, hi = k;  // run of slot e: last j with off[j] <= e
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (off[mid] <= e)
-        lo = mid;
-      else
-        hi = mid;
+  const uint32_t cnt = bnd[k];
+  {
+    // every load of the piece issued before the first LDS store
+    Bits v[kLoads];
+#pragma unroll
+    for (uint32_t i = 0; i < kLoads; ++i) {
+      const uint32_t e = (uint32_t)tid + i * kMwBlock;
+      v[i] = Bits(0);
+      if (e < cnt) {
+        uint32_t lo = 0, hi = k;  // the slice of slot e: last j with bnd[j] <= e
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (bnd[mid] <= e)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        v[i] = sptr[lo][sbeg[lo] + (e - bnd[lo])];
+      }
     }
-    const uint32_t j = lo;
-    const Bits x = in[e];
-    const Bits ex = Codec<KT>::enc(x);
-    uint32_t r = e - off[j];
-    for (uint32_t q = 0; q < j; ++q) r += bound_lds<KT, true>(in + off[q], off[q + 1] - off[q], ex);
-    for (uint32_t q = j + 1; q < k; ++q) r += bound_lds<KT, false>(in + off[q], off[q + 1] - off[q], ex);
-    res[min(r, kMwCap - 1)] = x;
+#pragma unroll
+    for (uint32_t i = 0; i < kLoads; ++i) {
+      const uint32_t e = (uint32_t)tid + i * kMwBlock;
+      if (e < cnt) buf[0][e] = v[i];
+    }
+  }
+  int cur = 0;
+  for (uint32_t slices = k; slices > 1; slices = (slices + 1) >> 1) {
+    __syncthreads();
+    const Bits* src = buf[cur];
+    Bits* dst = buf[cur ^ 1];
+    const uint32_t pairs = (slices + 1) >> 1;
+    uint32_t o = (uint32_t)tid * kSlots;
+    const uint32_t end = min(o + kSlots, cnt);
+    while (o < end) {
+      // the last pair i with bnd[2i] <= o: it ends beyond o (a later pair
+      // starts beyond o, or it is the last pair, which ends at cnt)
+      uint32_t lo = 0, hi = pairs;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bnd[2 * mid] <= o)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const uint32_t a0 = bnd[2 * lo], a1 = bnd[min(2 * lo + 1, slices)], b1 = bnd[min(2 * lo + 2, slices)];
+      const uint32_t na = a1 - a0, nb = b1 - a1;
+      const uint32_t pend = min(end, b1);
+      uint32_t ia = corank_lds<KT>(src + a0, na, src + a1, nb, o - a0);
+      uint32_t ib = (o - a0) - ia;
+      // the heads of both sides in registers: one LDS load per output
+      Bits ka = ia < na ? src[a0 + ia] : Bits(0);
+      Bits kb = ib < nb ? src[a1 + ib] : Bits(0);
+      for (; o < pend; ++o) {  // branch-free: one LDS store and one load per output
+        const bool take_a = ib >= nb || (ia < na && Codec<KT>::enc(ka) <= Codec<KT>::enc(kb));
+        dst[o] = take_a ? ka : kb;
+        ia += take_a ? 1u : 0u;
+        ib += take_a ? 0u : 1u;
+        const uint32_t at = take_a ? a0 + ia : a1 + ib;  // the taken side's next key (past its end: ignored)
+        const Bits nk = src[min(at, kMwCap - 1)];
+        ka = take_a ? nk : ka;
+        kb = take_a ? kb : nk;
+      }
+    }
+    __syncthreads();
+    // boundaries of the merged slices: slice i starts where pair i did
+    uint32_t nb_val = 0;
+    const uint32_t nslices = pairs;
+    if ((uint32_t)tid <= nslices) nb_val = bnd[min(2 * (uint32_t)tid, slices)];
+    __syncthreads();
+    if ((uint32_t)tid <= nslices) bnd[tid] = nb_val;
+    cur ^= 1;
   }
   __syncthreads();
   Bits* dst = out + bases[b];
-  for (uint32_t e = tid; e < cnt; e += kMwBlock) dst[e] = res[e];
+  for (uint32_t e = tid; e < cnt; e += kMwBlock) dst[e] = buf[cur][e];
 }
 
 uint32_t grid_of(uint64_t threads, int block) { return (uint32_t)((threads + block - 1) / block); }
@@ -253,6 +318,7 @@ int mw_launch(const void* const* runs, const size_t* lens, size_t k, void* out, 
   constexpr int KB = sizeof(Bits);
   const uint32_t S = sample_stride(k);
   const uint64_t ns = sample_count(lens, k, S);
+  if (ns > (uint64_t)MGS_MAX_KEYS) return MGS_ERR_TOO_LARGE;  // sample ids and the sample sort
   const MwLayout L = mw_layout(KB, k, ns);
   if (temp == nullptr || temp_bytes < L.total) return MGS_ERR_TEMP_TOO_SMALL;
   if (ns == 0) {  // no keys: nothing to merge, the status word says ok
@@ -298,7 +364,8 @@ int mw_launch(const void* const* runs, const size_t* lens, size_t k, void* out, 
                      sort_temp_bytes((mgs_key)KT, MGS_VAL_U32, ns), s);
   if (rc != MGS_OK) return rc;
   mw_split_kernel<KT, Bits><<<grid_of((L.pieces + 1) * k, 256), 256, 0, s>>>(d_runs, d_lens, d_sbase, (uint32_t)k, S,
-                                                                            sk2, sid2, L.pieces, splits);
+                                                                            sk2, sid2, L.pieces, piece_step(k),
+                                                                            splits);
   mw_base_kernel<<<grid_of(L.pieces + 1, 256), 256, 0, s>>>(splits, (uint32_t)k, L.pieces, bases, sort_status,
                                                             status);
   rc = check_launch();

@@ -18,9 +18,9 @@
 // the final merge (the reference merges a zero-filled buffer instead).
 //
 // merge_backend "gpu": the sorted chunks stay resident in HBM (when the
-// device has room for two copies of the input) and are combined by rounds of
-// pairwise merge-path merges, followed by one D2H; otherwise the CPU path
-// runs.
+// device has room for two copies of the input) and are combined by one k-way
+// merge (mgs_multiway_merge), whose output goes back D2H in stretches while
+// the later stretches merge; otherwise the CPU path runs.
 #pragma once
 
 #include <cstddef>
