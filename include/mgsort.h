@@ -118,8 +118,8 @@ int mgs_sort(mgs_key kt, mgs_val vt,
  * callers that cannot allocate device memory: code under HIP graph capture,
  * tight serving loops).  Same kernels as mgs_sort; the scratch is a 64 MiB
  * array in the library's device memory, one per GPU, held until the process
- * ends, so n is limited to mgs_sort_temp_bytes(kt, vt, n) <= 64 MiB (about
- * 15.9M u32 keys, 7.9M u64 keys or u32 pairs, 5.2M u64 keys with u32
+ * ends, so n is limited to mgs_sort_temp_bytes(kt, vt, n) <= 64 MiB (2^23 - 1
+ * u32 keys, about 8.3M u64 keys or u32 pairs, 5.5M u64 keys with u32
  * payloads); larger n returns MGS_ERR_TOO_LARGE.  Allocates nothing.  All
  * calls on one GPU share its array: calls that overlap on one GPU (on
  * different streams or host threads) race and must be ordered by the
@@ -146,8 +146,9 @@ int mgs_sort_timed(mgs_key kt, mgs_val vt,
  * `stream` and returns MGS_PATH_SMALL (one workgroup in LDS, no scratch),
  * MGS_PATH_LSD (onesweep LSD passes) or MGS_PATH_THREE_PASS (the MSD pass,
  * the bucket pass and the LDS sort of sub-buckets, tried for 4-byte keys
- * alone from 2^26 keys; it hands over to the LSD passes when the key
- * distribution does not suit it), or a negative status. */
+ * alone of 2^23 to 3 * 2^24 keys and of 2^29 keys and more; it hands over to
+ * the LSD passes when the key distribution does not suit it), or a negative
+ * status. */
 enum { MGS_PATH_SMALL = 1, MGS_PATH_LSD = 2, MGS_PATH_THREE_PASS = 3 };
 int mgs_sort_path(mgs_key kt, mgs_val vt, size_t n, const void* temp, size_t temp_bytes, void* stream);
 

@@ -1,7 +1,8 @@
 // Three-pass hybrid radix sort for large arrays of 4-byte keys (u32, i32,
-// f32; keys alone), the path of mgs_sort for the in-HBM sorts of 2^26 keys
-// and more.  The plain LSD sort (radix_sort.hip) reads and writes the array
-// four times; this one does it three times:
+// f32; keys alone), the path of mgs_sort for the in-HBM sorts of 2^23 to
+// 3 * 2^24 keys and of 2^29 keys and more (msd_eligible).  The plain LSD
+// sort (radix_sort.hip) reads and writes the array four times; this one
+// does it three times:
 //
 //   upfront  one read: every workgroup counts the top 16 bits of the encoded
 //            keys of its range (its pass-1 chain) in a 65536-bin LDS
@@ -740,8 +741,10 @@ uint32_t msd_resident(K kernel) {
 }  // namespace
 
 bool msd_eligible(size_t n) {
-  return n >= env_u32("MGS_MSD_MIN", MsdCfg::kMinKeys) && n <= MsdCfg::kMaxKeys &&
-         env_u32("MGS_MSD", 1) != 0;
+  if (env_u32("MGS_MSD", 1) == 0 || n > MsdCfg::kMaxKeys) return false;
+  const char* e = std::getenv("MGS_MSD_MIN");
+  if (e != nullptr && e[0] != 0) return n >= env_u32("MGS_MSD_MIN", 0);
+  return (n >= MsdCfg::kMinKeys && n < MsdCfg::kGapLo) || n >= MsdCfg::kGapHi;
 }
 
 // Scratch of the path (at most MsdCfg::kMaxChains upfront ranges) from

@@ -22,10 +22,16 @@ struct MsdCfg {
   static constexpr int kLocalIpt = 20;
   static constexpr uint32_t kLocalCap = (uint32_t)kBlock * kLocalIpt;
   static constexpr uint32_t kMaxChains = 256;
-  // sizes the path is tried for (MGS_MSD_MIN overrides the lower bound):
-  // from 2^26 keys, and up to where the mean sub-bucket (n / 65536) still
-  // leaves room below kLocalCap
-  static constexpr size_t kMinKeys = size_t(1) << 26;
+  // sizes the path is tried for by default: [kMinKeys, kGapLo) and
+  // [kGapHi, kMaxKeys], where it measured faster than the LSD sort on
+  // uniform u32 and f32 keys (2^23-2^25: 2-9%; 2^29: 1%; 2^30: 21%); in
+  // between, batches of several small sub-buckets take a third LDS pass and
+  // the LSD sort is faster (2^26: 3%, 2^28: 12%).  MGS_MSD_MIN replaces both
+  // ranges by [MGS_MSD_MIN, kMaxKeys].  kMaxKeys: the mean sub-bucket
+  // (n / 65536) still leaves room below kLocalCap.
+  static constexpr size_t kMinKeys = size_t(1) << 23;
+  static constexpr size_t kGapLo = size_t(3) << 24;
+  static constexpr size_t kGapHi = size_t(1) << 29;
   static constexpr size_t kMaxKeys = (size_t(1) << 30) + (size_t(1) << 27);
 };
 

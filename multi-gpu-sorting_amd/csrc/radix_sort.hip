@@ -339,7 +339,7 @@ __device__ __forceinline__ uint32_t chain_digit_base(const uint32_t* __restrict_
 //      lane's digit take one LDS atomic instead of one each), second of the
 //      pair: exits at once unless the pass is skewed;
 //   3  peer-mask ranking (rank_wave), chosen at run time with
-//      MGS_RANK_PEERS=1 for 4-byte keys.
+//      MGS_RANK_PEERS=1 for 4-byte keys (single-round passes).
 // Variants 0-2 are stable because the LDS serves the conflicting lanes of one
 // ds_add_rtn_u32 in ascending lane order: measured on gfx950 (and re-checked
 // by every multi-pass parity test with payloads), not a documented guarantee.
@@ -358,13 +358,23 @@ __device__ __forceinline__ uint32_t chain_digit_base(const uint32_t* __restrict_
 // after it, so the look-back latency overlaps the scan and the scatter.
 // Every wavefront loads its part of the next tile as soon as its keys of the
 // current tile are in LDS.
-template <int KT, typename V, int BLOCK, int IPT, typename D, int WAVES, int RANK = 0>
+// MULTI: the pass runs in rounds (descriptor budget exceeded): tile numbers
+// inside a round and the digit counts carried between rounds.  Single-round
+// passes compile that bookkeeping out (measured 5% per pass on 2^29 u64 keys).
+template <int KT, typename V, int BLOCK, int IPT, typename D, int WAVES, int RANK = 0, bool MULTI = false>
 __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     const typename Codec<KT>::Bits* k_in, typename Codec<KT>::Bits* k_alt, typename Codec<KT>::Bits* k_out,
     const V* v_in, V* v_alt, V* v_out, uint32_t n, int pass, int aliased, ChainMap cm,
     const uint64_t* __restrict__ red, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters,
     D* __restrict__ desc, uint32_t* __restrict__ status, uint32_t spin_limit, const uint32_t* __restrict__ gate,
-    PassRound rd) {
+    PassRound rd_arg) {
+  PassRound rd = rd_arg;
+  if constexpr (!MULTI) {
+    rd.first_t = 0;
+    rd.rounds = 1;
+    rd.carry_in = nullptr;
+    rd.carry_out = nullptr;
+  }
   using Bits = typename Codec<KT>::Bits;
   constexpr int P = (int)sizeof(Bits);
   constexpr int W = BLOCK / kWave;
@@ -815,7 +825,11 @@ template <int KB, int VB>
 size_t sort_bytes(size_t n) {
   const SortLayout L = sort_layout<KB, VB>(n);
   size_t b = L.total;
-  if (KB == 4 && VB == 0 && msd_eligible(n)) b = std::max(b, msd_layout(n, L.desc, L.msd_ctl).end);
+  // the three-pass path's region wherever it may run, and from its lowest
+  // default size on whatever the size ranges, so that the answer does not
+  // decrease with n
+  if (KB == 4 && VB == 0 && (msd_eligible(n) || n >= MsdCfg::kMinKeys))
+    b = std::max(b, msd_layout(std::min(n, MsdCfg::kMaxKeys), L.desc, L.msd_ctl).end);
   if (KB + VB > 4 && n >= kBigTileMin) b = std::max(b, sort_layout_cfg<KB, VB, false>(kBigTileMin - 1).total);
   return b;
 }
@@ -931,6 +945,12 @@ int sort_typed_cfg(const void* kin, void* kout, const void* vin, void* vout, siz
   const bool peers = KB + VB == 4 && rank_peers_requested();
   auto launch_ranked = [&](auto* dsc, int p) {
     using DT = std::remove_pointer_t<decltype(dsc)>;
+    if (L.rounds > 1) {
+      // passes in rounds (beyond the descriptor budget: about 1.4e9 keys and
+      // more): one atomic per key, whatever the byte's skew
+      launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, DT, Cfg::kWavesPerEu, 0, true>, dsc, p);
+      return;
+    }
     if constexpr (KB + VB == 4) {
       if (peers) {
         launch_pass(pass_kernel<KT, V, Cfg::kBlock, Cfg::kIpt, DT, Cfg::kWavesPerEu, 3>, dsc, p);

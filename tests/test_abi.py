@@ -74,12 +74,15 @@ PAIRS = [(k, v) for k in range(6) for v in range(3)]
 def test_sort_temp_bytes_never_decrease(kt, vt):
     """A buffer sized for n serves every smaller sort: the scratch query does
     not decrease across the small-sort limit, the switch to large tiles at
-    2^21 keys for records wider than 4 bytes, and up to MGS_MAX_KEYS."""
+    2^21 keys for records wider than 4 bytes, the size ranges of the
+    three-pass path (2^23, 3 * 2^24, 2^29, 2^30 + 2^27) and up to
+    MGS_MAX_KEYS."""
     import mgsort
     L = mgsort.lib
     ns = sorted({1, 2047, 2048, 4095, 4096, 4097, 8191, 8192, 8193, (1 << 21) - 10000, (1 << 21) - 1, 1 << 21,
-                 (1 << 21) + 1, (1 << 21) + 10000, 1 << 24, (1 << 26) - 1, 1 << 26, 1 << 30, (1 << 30) + 1,
-                 mgsort.MAX_KEYS - 1, mgsort.MAX_KEYS})
+                 (1 << 21) + 1, (1 << 21) + 10000, (1 << 23) - 1, 1 << 23, 1 << 24, (3 << 24) - 1, 3 << 24,
+                 (1 << 26) - 1, 1 << 26, (1 << 29) - 1, 1 << 29, 1 << 30, (1 << 30) + 1,
+                 (1 << 30) + (1 << 27), (1 << 30) + (1 << 27) + 1, mgsort.MAX_KEYS - 1, mgsort.MAX_KEYS})
     sizes = [L.mgs_sort_temp_bytes(kt, vt, n) for n in ns]
     assert all(b >= a for a, b in zip(sizes, sizes[1:])), list(zip(ns, sizes))
     # the alternate buffers are in the scratch of every sort that needs scratch

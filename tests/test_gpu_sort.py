@@ -254,13 +254,22 @@ def test_lookback_bound_reports_status(mg, kt, vdt, monkeypatch):
 MSD_KEYS = [O.K_U32, O.K_I32, O.K_F32]
 
 
-def expected_path(kt, keys):
+def msd_default_size(n):
+    """Sizes the three-pass path is tried for without MGS_MSD_MIN:
+    [2^23, 3 * 2^24) and [2^29, 2^30 + 2^27]."""
+    return (1 << 23) <= n < (3 << 24) or (1 << 29) <= n <= (1 << 30) + (1 << 27)
+
+
+def expected_path(kt, keys, forced=True):
     """The three-pass path hands over to the LSD passes when a 16-bit
     sub-bucket exceeds the 20480 keys of the LDS sort or a byte-3 bucket
-    holds more than twice its share plus two 18432-key tiles."""
+    holds more than twice its share plus two 18432-key tiles.  forced: the
+    path is tried at any size (MGS_MSD_MIN=0)."""
     n = keys.size
     if n <= 4096:
         return 1
+    if not forced and not msd_default_size(n):
+        return 2
     top = (O.encode(kt, keys) >> np.uint64(16)).astype(np.int64)
     sub = np.bincount(top, minlength=1 << 16)
     bucket = sub.reshape(256, 256).sum(axis=1)
@@ -298,12 +307,15 @@ def test_three_pass_path(mg, kt, n, dist, monkeypatch):
             assert np.array_equal(from_dev(dk, keys.dtype), keys)
 
 
-@pytest.mark.parametrize("dist", ["uniform", "sorted", "reverse", "skewed"])
-def test_three_pass_default_size(mg, dist):
-    """From 2^26 keys the path is on by default (no environment)."""
+@pytest.mark.parametrize("dist,n", [("uniform", (1 << 24) + 12_345), ("sorted", (1 << 24) + 12_345),
+                                    ("reverse", (1 << 24) + 12_345), ("skewed", (1 << 24) + 12_345),
+                                    ("uniform", (1 << 26) + 12_345)])
+def test_three_pass_default_size(mg, dist, n):
+    """Without environment the path is tried in [2^23, 3 * 2^24) and from
+    2^29 keys (2^26 takes the LSD passes); the output is the oracle's either
+    way."""
     import torch
     kt = O.K_U32
-    n = (1 << 26) + 12_345
     rng = np.random.default_rng(26)
     keys = random_keys(kt, n, rng, dist)
     ek, _ = O.sort(kt, keys)
@@ -312,7 +324,7 @@ def test_three_pass_default_size(mg, dist):
     mg.sort("u32", dk, temp=temp)
     assert mg.temp_status(temp) == 0
     path = mg.lib.mgs_sort_path(kt, mg.VAL_NONE, n, mg._ptr(temp), temp.numel(), None)
-    assert path == expected_path(kt, keys)
+    assert path == expected_path(kt, keys, forced=False)
     assert np.array_equal(from_dev(dk, np.uint32), ek)
 
 
