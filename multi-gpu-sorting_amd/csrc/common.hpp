@@ -103,15 +103,19 @@ __device__ __forceinline__ uint32_t readfirstlane(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
-// Inclusive prefix sum over the 64 lanes of a wavefront (DPP row shifts plus
-// two cross-row steps through ds_swizzle-free permutes).
+// Inclusive prefix sum over the 64 lanes of a wavefront, in DPP moves (no
+// LDS): row_shr 1, 2, 4, 8 scan each row of 16 lanes (a source lane outside
+// the row reads as 0), then row_bcast:15 adds row 0's total to row 1 and row
+// 2's to row 3, and row_bcast:31 adds the first two rows' total to rows 2
+// and 3 (row masks 0xA and 0xC leave the other rows' values at 0).  Every
+// lane of the wavefront must be active.
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    uint32_t t = __shfl_up(v, off, kWave);
-    if (lane >= off) v += t;
-  }
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return v;
 }
 
