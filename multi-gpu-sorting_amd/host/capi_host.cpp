@@ -20,6 +20,7 @@
 #include "key_traits.h"
 #include "device_buffer.h"
 #include "hip_check.h"
+#include "inplace_transfer.h"
 #include "paradis.h"
 #include "seg_tuning.h"
 #include "sort_server.h"
@@ -252,14 +253,8 @@ int mgh_link_probe(int gpu, int mode, size_t bytes, size_t block_bytes, int reps
         HIP_CHECK(hipMemcpyAsync(dp, h_in.data(), bytes, hipMemcpyHostToDevice, s1));
         HIP_CHECK(hipMemcpyAsync(h_out.data(), d2.get(), bytes, hipMemcpyDeviceToHost, s2));
       } else {
-        size_t k = 0;
-        for (size_t at = 0; at < bytes; at += block, ++k) {
-          const size_t b = std::min(block, bytes - at);
-          HIP_CHECK(hipMemcpyAsync(h_out.data() + at, dp + at, b, hipMemcpyDeviceToHost, s2));
-          out_ev[k]->Record(s2);
-          HIP_CHECK(hipStreamWaitEvent(s1, *out_ev[k], 0));
-          HIP_CHECK(hipMemcpyAsync(dp + at, h_in.data() + at, b, hipMemcpyHostToDevice, s1));
-        }
+        mgsort::EnqueueInplaceTransfer(dp, reinterpret_cast<const char*>(h_in.data()), bytes,
+                                       reinterpret_cast<char*>(h_out.data()), bytes, block, s1, s2, out_ev);
       }
       ev.Record(s2);
       HIP_CHECK(hipStreamWaitEvent(s1, ev, 0));

@@ -15,6 +15,7 @@
 #include "chunk_merger.h"
 #include "device_buffer.h"
 #include "hip_check.h"
+#include "inplace_transfer.h"
 #include "key_traits.h"
 #include "time_durations.h"
 
@@ -185,21 +186,7 @@ void GowanlockSort(HostVector<T>* elements, size_t chunk_size, size_t merge_grou
       const char* in_src = c + 1 < C ? reinterpret_cast<const char*>(primary + (c + 1) * cs) : nullptr;
       char* out_dst = c >= 1 ? reinterpret_cast<char*>(run_dst(c - 1)) : nullptr;
       if (c >= 1) HIP_CHECK(hipStreamWaitEvent(d2h, *sort_done[c - 1], 0));
-      const size_t both = std::min(out_bytes, in_bytes);
-      size_t k = 0;
-      for (size_t at = 0; at < std::max(out_bytes, in_bytes); at += block, ++k) {
-        const size_t ob = at < out_bytes ? std::min(block, out_bytes - at) : 0;
-        const size_t ib = at < in_bytes ? std::min(block, in_bytes - at) : 0;
-        if (ob) {
-          HIP_CHECK(hipMemcpyAsync(out_dst + at, y + at, ob, hipMemcpyDeviceToHost, d2h));
-          block_out[k]->Record(d2h);
-        }
-        if (ib) {
-          // the block's old contents must be out first (where chunk c-1 had them)
-          if (at < both || (ob && at < out_bytes)) HIP_CHECK(hipStreamWaitEvent(h2d, *block_out[k], 0));
-          HIP_CHECK(hipMemcpyAsync(y + at, in_src + at, ib, hipMemcpyHostToDevice, h2d));
-        }
-      }
+      EnqueueInplaceTransfer(y, in_src, in_bytes, out_dst, out_bytes, block, h2d, d2h, block_out);
       if (c >= 1) d2h_done[c - 1]->Record(d2h);
       if (c + 1 < C) {
         // chunk c+1 may be longer than chunk c-1 was: its blocks past the

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "argument_limits.h"
+#include "bench_common.h"
 #include "data_generator.h"
 #include "device_buffer.h"
 #include "gowanlock_sort.h"
@@ -70,31 +71,6 @@ struct Options {
   int num_threads = 1;
   size_t chunk_size = 0, merge_group_size = 1, num_buffers = 2, partitions = 0, block_size = 0;
 };
-
-// Order-independent fingerprint of a multiset of keys (sum and xor of a
-// mixed image of every key).
-template <typename T>
-std::pair<uint64_t, uint64_t> Fingerprint(const T* a, size_t n) {
-  uint64_t s = 0, x = 0;
-#pragma omp parallel for reduction(+ : s) reduction(^ : x)
-  for (size_t i = 0; i < n; ++i) {
-    uint64_t v = (uint64_t)Encode(a[i]);
-    v ^= v >> 31;
-    v *= 0x9E3779B97F4A7C15ull;
-    v ^= v >> 29;
-    s += v;
-    x ^= v;
-  }
-  return {s, x};
-}
-
-template <typename T>
-bool IsSortedEncoded(const T* a, size_t n) {
-  bool ok = true;
-#pragma omp parallel for reduction(&& : ok)
-  for (size_t i = 1; i < n; ++i) ok = ok && !(Encode(a[i]) < Encode(a[i - 1]));
-  return ok;
-}
 
 // The "thrust" branch (reference :81-91): H2D, sort, D2H inside sort_phase.
 // The "hip" branch: in-HBM sort, copies outside the timed phase.
