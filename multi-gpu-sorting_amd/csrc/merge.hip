@@ -16,9 +16,16 @@ namespace mgs {
 
 namespace {
 
-constexpr int kMergeBlock = 256;
-constexpr int kMergeIpt = 15;  // odd: staggers LDS banks of the serial merges
+#ifndef MGS_MERGE_BLOCK
+#define MGS_MERGE_BLOCK 256
+#endif
+#ifndef MGS_MERGE_IPT
+#define MGS_MERGE_IPT 15
+#endif
+constexpr int kMergeBlock = MGS_MERGE_BLOCK;
+constexpr int kMergeIpt = MGS_MERGE_IPT;  // odd: staggers LDS banks of the serial merges
 constexpr int kMergeTile = kMergeBlock * kMergeIpt;
+static_assert(kMergeIpt % 2 == 1 && kMergeTile >= 3840, "odd items per thread; tiles of at least 3840 keys");
 
 // Number of A elements among the first `diag` outputs of the stable merge.
 template <int KT, typename Bits>
@@ -72,15 +79,26 @@ __global__ __launch_bounds__(kMergeBlock) void merge_tile_kernel(const typename 
   const int la = (int)(a1 - a0);
   const int lb = cnt - la;
 
-  for (int i = tid; i < cnt; i += kMergeBlock) {
-    if (i < la) {
-      sk[i] = a[a0 + i];
-      if constexpr (HV) sv[i] = av[a0 + i];
-    } else {
-      const uint64_t j = b0 + (uint64_t)(i - la);
-      const bool ok = j < nb;
-      sk[i] = ok ? b[j] : Bits(0);
-      if constexpr (HV) sv[i] = ok ? bv[j] : VS(0);
+  // staging: every load of the thread is issued before the first LDS store
+  {
+    Bits lk[kMergeIpt];
+    VS lv[kMergeIpt];
+#pragma unroll
+    for (int j = 0; j < kMergeIpt; ++j) {
+      const int i = j * kMergeBlock + tid;
+      const uint64_t jb = b0 + (uint64_t)(i - la);
+      const bool from_a = i < la;
+      const bool ok = i < cnt && (from_a || jb < nb);
+      lk[j] = ok ? (from_a ? a[a0 + i] : b[jb]) : Bits(0);
+      if constexpr (HV) lv[j] = ok ? (from_a ? av[a0 + i] : bv[jb]) : VS(0);
+    }
+#pragma unroll
+    for (int j = 0; j < kMergeIpt; ++j) {
+      const int i = j * kMergeBlock + tid;
+      if (i < cnt) {
+        sk[i] = lk[j];
+        if constexpr (HV) sv[i] = lv[j];
+      }
     }
   }
   __syncthreads();
@@ -134,9 +152,22 @@ __global__ __launch_bounds__(kMergeBlock) void merge_tile_kernel(const typename 
     }
   }
   __syncthreads();
-  for (int i = tid; i < cnt; i += kMergeBlock) {
-    out[d0 + i] = sk[i];
-    if constexpr (HV) ov[d0 + i] = sv[i];
+  // write-out: every LDS read of the thread first, then the stores
+#pragma unroll
+  for (int j = 0; j < kMergeIpt; ++j) {
+    const int i = j * kMergeBlock + tid;
+    if (i < cnt) {
+      rk[j] = sk[i];
+      if constexpr (HV) rv[j] = sv[i];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kMergeIpt; ++j) {
+    const int i = j * kMergeBlock + tid;
+    if (i < cnt) {
+      out[d0 + i] = rk[j];
+      if constexpr (HV) ov[d0 + i] = rv[j];
+    }
   }
 }
 

@@ -4,24 +4,32 @@
 // sort (radix_sort.hip) reads and writes the array four times; this one
 // does it three times:
 //
-//   upfront  one read: every workgroup counts the top 16 bits of the encoded
-//            keys of its range (its pass-1 chain) in a 65536-bin LDS
-//            histogram of 16-bit counters, and stores it as its slice
+//   upfront  one read: every workgroup counts a 16-bit window of the
+//            encoded keys of its range (its pass-1 chain) in a 65536-bin LDS
+//            histogram of 16-bit counters, and stores it as its slice; the
+//            window is the top 16 bits below the leading bits that a sample
+//            of 1024 keys shares (the top 16 bits of keys that use the whole
+//            range; bits 15..30 of non-negative int keys), and a key outside
+//            the sample's common leading bits turns the path off
 //   scan     per top byte b3: the sub-bucket sizes (b3, b2) summed over the
 //            slices, their offsets inside bucket b3, the chains' prefix of
 //            every b3 digit, and the batches of the local sort; a bucket or
 //            sub-bucket too large for the local sort, or a 16-bit counter
 //            that overflowed, turns the path off (the LSD sort runs instead)
 //   finish   one workgroup: bucket starts, batch list offsets, tile counters
-//   pass 1   MSD scatter by byte 3: workgroup c streams chain c (a fixed
+//   pass 1   MSD scatter by the window's high byte ("byte 3" below; byte 3
+//            itself for full-range keys): workgroup c streams chain c (a fixed
 //            range of the input) tile by tile with running per-digit output
 //            offsets -- the chain's digit bases come from the upfront
 //            histograms, so there is no look-back between workgroups
-//   pass 2   scatter by byte 2 inside each byte-3 bucket: workgroup b streams
+//   pass 2   scatter by the window's low byte ("byte 2") inside each byte-3
+//            bucket: workgroup b streams
 //            bucket b the same way, bases from the sub-bucket offsets
 //   local    every batch of consecutive sub-buckets (at most kLocalCap keys,
-//            one top-16-bit value, or several of one bucket) is read once,
-//            sorted by its low bytes in LDS and written back in place
+//            one window value, or several of one bucket) is read once,
+//            sorted by its low 16 bits (one sub-bucket: its keys agree on
+//            every bit above them) or 24 bits (one bucket) in LDS and written
+//            back in place
 //
 // Load balance: a workgroup that runs out of work takes tiles from the back
 // end of another chain (at most one such helper per chain); a back-end
@@ -56,7 +64,6 @@ constexpr uint32_t kNoWork = 0xFFFFFFFFu;
 // Decision word bits (MsdCtl::off): any set bit turns the path off.
 enum : uint32_t { kOffOverflow = 1u, kOffSubBucket = 2u, kOffBucket = 4u };
 
-__device__ __forceinline__ uint32_t top16(uint32_t enc) { return enc >> 16; }
 
 // ---------------------------------------------------------------------------
 // upfront
@@ -73,16 +80,50 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
   // 65536 16-bit counters, two per word: bin b in word b >> 1, half b & 1
   __shared__ uint32_t h[kMsdBins / 2];
   __shared__ uint32_t s_rows[kRadix];
-  __shared__ uint32_t s_runs;
+  __shared__ uint32_t s_runs, s_stray;
+  __shared__ uint32_t s_and[kMsdBlock / kWave], s_or[kMsdBlock / kWave];
   const int tid = threadIdx.x;
   const uint32_t c = blockIdx.x;
+  const int lane = tid & (kWave - 1), w = tid / kWave;
+  for (int i = tid; i < kMsdBins / 2; i += kMsdBlock) h[i] = 0;
+  if (tid == 0) {
+    xcd_of[c] = xcc_id();
+    s_stray = 0;
+  }
+  // the window: every workgroup reads the same sample of 1024 keys, evenly
+  // spaced over the input (n >= 2), and derives the same bits from it
+  {
+    const uint32_t e = Codec<KT>::enc(keys[((uint64_t)tid * (n - 1)) / (kMsdBlock - 1)]);
+    uint32_t ea = e, eo = e;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ea &= __shfl_xor(ea, o, kWave);
+      eo |= __shfl_xor(eo, o, kWave);
+    }
+    if (lane == 0) {
+      s_and[w] = ea;
+      s_or[w] = eo;
+    }
+  }
+  __syncthreads();
+  uint32_t s_a = ~0u, s_o = 0u;
+#pragma unroll
+  for (int i = 0; i < kMsdBlock / kWave; ++i) {
+    s_a &= s_and[i];
+    s_o |= s_or[i];
+  }
+  const uint32_t common = (s_a ^ s_o) ? (uint32_t)__clz(s_a ^ s_o) : 32u;  // leading bits the sample shares
+  const uint32_t low = 16u - min(common, 16u);
+  const uint32_t hi_mask = low < 16u ? ~0u << (low + 16u) : 0u;  // the shared bits above the window
+  const uint32_t hi_val = s_a & hi_mask;
   if (c == 0 && tid == 0) {  // nothing else writes these during this kernel
     *status = 0;
     ctl->off = 0;
     ctl->xcd_mask = 0;
+    ctl->low = low;
   }
-  for (int i = tid; i < kMsdBins / 2; i += kMsdBlock) h[i] = 0;
-  if (tid == 0) xcd_of[c] = xcc_id();
+  auto window = [&](uint32_t enc) { return (enc >> low) & 0xFFFFu; };
+  uint32_t stray = 0;  // bits above the window where a key differs from the sample
   {
     // look-back descriptors of both passes: the tiles of the input read as
     // "empty" by pass 1 (status rotation 0) and, once pass 1 has left them
@@ -104,15 +145,23 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
   auto add = [&](uint32_t b, uint32_t k) {
     __hip_atomic_fetch_add(&h[b >> 1], k << ((b & 1u) << 4), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   };
-  auto count = [&](uint32_t k) { add(top16(Codec<KT>::enc(k)), 1u); };
+  auto count = [&](uint32_t k) {
+    const uint32_t e = Codec<KT>::enc(k);
+    stray |= (e & hi_mask) ^ hi_val;
+    add(window(e), 1u);
+  };
   // four consecutive keys per lane: one add when they share a bin, and one
   // add for all the lanes whose four keys fall in the first lane's bin (runs
   // of sorted keys, which would otherwise put every lane of the wavefront on
   // one LDS counter); such keys are counted in `run_keys`
   uint32_t run_keys = 0;
   auto count4 = [&](const uint4& v) {
-    const uint32_t b0 = top16(Codec<KT>::enc(v.x)), b1 = top16(Codec<KT>::enc(v.y));
-    const uint32_t b2 = top16(Codec<KT>::enc(v.z)), b3 = top16(Codec<KT>::enc(v.w));
+    const uint32_t e0 = Codec<KT>::enc(v.x), e1 = Codec<KT>::enc(v.y);
+    const uint32_t e2 = Codec<KT>::enc(v.z), e3 = Codec<KT>::enc(v.w);
+    stray |= ((e0 & hi_mask) ^ hi_val) | ((e1 & hi_mask) ^ hi_val) | ((e2 & hi_mask) ^ hi_val) |
+             ((e3 & hi_mask) ^ hi_val);
+    const uint32_t b0 = window(e0), b1 = window(e1);
+    const uint32_t b2 = window(e2), b3 = window(e3);
     const bool same = (b0 == b1) & (b0 == b2) & (b0 == b3);
     const bool in = same && b0 == readfirstlane(b0);
     const uint64_t grp = ballot64(in);
@@ -152,8 +201,8 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
   if (tid == 0) s_runs = 0;
   __syncthreads();
   if (run_keys) atomicAdd(&s_runs, run_keys);
+  if (stray) s_stray = 1u;
   // byte-3 histogram of the chain: row r is the 128 words [128 r, 128 r + 128)
-  const int lane = tid & (kWave - 1), w = tid / kWave;
   for (int r = w; r < kRadix; r += kMsdBlock / kWave) {
     const uint32_t a = h[r * 128 + lane], b = h[r * 128 + 64 + lane];
     uint32_t s = (a & 0xFFFFu) + (a >> 16) + (b & 0xFFFFu) + (b >> 16);
@@ -166,11 +215,13 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
   if (tid < kWave) {
     // a wrapped 16-bit counter loses 65536 (high half) or 65535 (low half,
     // whose carry lands in the high half): the rows then fall short of the
-    // range, and the path is turned off
+    // range, and the path is turned off; so it is by a key outside the
+    // sample's shared leading bits
     uint32_t s = s_rows[tid] + s_rows[tid + 64] + s_rows[tid + 128] + s_rows[tid + 192];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
-    if (tid == 0) chain_bad[c] = (s != (uint32_t)(hi > lo ? hi - lo : 0) ? 1u : 0u) | (s_runs << 1);
+    if (tid == 0)
+      chain_bad[c] = (s != (uint32_t)(hi > lo ? hi - lo : 0) || s_stray ? 1u : 0u) | (s_runs << 1);
   }
   uint4* dst = reinterpret_cast<uint4*>(slices + (size_t)c * (kMsdBins / 2));
   for (int i = tid; i < kMsdBins / 8; i += kMsdBlock)
@@ -328,7 +379,6 @@ struct MsdPassArgs {
   const uint32_t* tile_base;   // first descriptor tile of each chain
   uint32_t* counters;          // tiles taken per chain
   uint32_t* desc;              // look-back descriptors, 256 per tile
-  int shift;                   // 24 or 16
   int parity;                  // descriptor status rotation of this pass
   const MsdCtl* ctl;
   uint32_t* status;
@@ -350,6 +400,7 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
   using D = uint32_t;
   if (a.ctl->off) return;
   const bool runs = a.ctl->runs != 0;
+  const int shift = (int)a.ctl->low + (PASS == 1 ? 8 : 0);  // the window's high or low byte
 
   __shared__ uint32_t keys[TILE];
   __shared__ uint32_t off[W][kRadix];
@@ -435,7 +486,7 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
     if (full) {
       uint32_t dg[IPT];
 #pragma unroll
-      for (int i = 0; i < IPT; ++i) dg[i] = digit_of<KT>(kc[i], a.shift);
+      for (int i = 0; i < IPT; ++i) dg[i] = digit_of<KT>(kc[i], shift);
       if (runs)
         rank_items_full<IPT, true>(dg, ct, pk);
       else
@@ -443,7 +494,7 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
     } else {
 #pragma unroll
       for (int i = 0; i < IPT; ++i) {
-        const uint32_t d = digit_of<KT>(kc[i], a.shift);
+        const uint32_t d = digit_of<KT>(kc[i], shift);
         pk[i] = (wb + i * kWave < chi) ? ((d << 16) | rank_wave_atomic<false>(d, ct)) : 0u;
       }
     }
@@ -526,7 +577,7 @@ __global__ __launch_bounds__(kMsdBlock, 4) void msd_pass_kernel(MsdPassArgs a) {
       const uint32_t i = (uint32_t)(j * kMsdBlock + tid);
       if (full || i < tile_cnt) {
         const uint32_t kk = keys[i];
-        const uint32_t o = min(gbase[digit_of<KT>(kk, a.shift)] + i, a.n - 1);
+        const uint32_t o = min(gbase[digit_of<KT>(kk, shift)] + i, a.n - 1);
         a.dst[o] = kk;
       }
     }
@@ -853,7 +904,6 @@ int msd_launch(const void* kin, void* kout, void* alt, size_t n, char* t, const 
   a.chains = chains1;
   a.tile_base = tile_base1;
   a.counters = counters + kRadix;
-  a.shift = 24;
   a.parity = 0;
   msd_pass_kernel<KT, 1><<<std::min(grid, tiles), kMsdBlock, 0, s>>>(a);
   mark(s, "msd_pass2");
@@ -862,7 +912,6 @@ int msd_launch(const void* kin, void* kout, void* alt, size_t n, char* t, const 
   a.chains = kRadix;
   a.tile_base = tile_base2;
   a.counters = counters;
-  a.shift = 16;
   a.parity = 1;
   msd_pass_kernel<KT, 2><<<grid, kMsdBlock, 0, s>>>(a);
   mark(s, "msd_batches");

@@ -43,12 +43,17 @@ struct MsdCfg {
 // sixteenth of the keys shared their top 16 bits with the first lane of
 // their wavefront load in the upfront (runs of sorted keys, where most lanes
 // of a wavefront would hit one counter).
+// low: the sub-buckets are bits [low, low + 16) of the encoded key, where
+// 16 - low is the number of leading bits (at most 16) that a sample of the
+// keys shares; every key must share them, or the path is off.  Pass 1
+// scatters by bits [low + 8, low + 16), pass 2 by bits [low, low + 8).
 struct MsdCtl {
   uint32_t off;
   uint32_t batches;
   uint32_t batch_next;
   uint32_t xcd_mask;  // XCDs the upfront kernel ran on
   uint32_t runs;
+  uint32_t low;
 };
 
 struct MsdLayout {

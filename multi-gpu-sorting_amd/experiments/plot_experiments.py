@@ -122,6 +122,7 @@ def aggregate(ident: str, df: pandas.DataFrame) -> pandas.DataFrame:
     extra = ["bytes_moved"] if "bytes_moved" in df.columns else []
     g = df.groupby(keys + extra, as_index=False)[p.y].agg(["mean", "min", "max", "count"])
     g = g.reset_index() if keys[0] not in g.columns else g
+    g = g.sort_values(keys + extra).reset_index(drop=True)
     if p.rate == "keys":
         g["gkeys_per_s"] = g[p.x] / g["mean"] / 1e9
     elif p.rate == "keys2":  # two inputs of x keys each

@@ -5,8 +5,11 @@ per argument tuple, the header line written once per file.
 
 Differences from the reference's runner:
 * each run has a time limit (--timeout) and its exit status is checked: a
-  failed run is reported with the tail of its output, not appended, and makes
-  the runner exit 1 after the grid (the reference drops failures silently);
+  run that fails with exit status 1 (the programs' error exit) is reported
+  with the tail of its output, not appended, and makes the runner exit 1
+  after the grid (the reference drops failures silently); a run that hits
+  its time limit, dies by a signal or exits otherwise stops the grid at once
+  (exit 3);
 * every output line is checked against the experiment's column count;
 * --scale multiplies the key and byte counts, --max-elements skips runs above
   a key count, --repetitions overrides the repetitions, --dry-run prints the
@@ -100,6 +103,8 @@ def main(argv=None) -> int:
     ap.add_argument("--repetitions", type=int, default=None, help="runs per argument tuple (default: the grid's)")
     ap.add_argument("--scale", type=float, default=1.0, help="factor on key and byte counts")
     ap.add_argument("--max-elements", type=float, default=None, help="skip runs above this many keys or bytes")
+    ap.add_argument("--where", action="append", default=[],
+                    help="keep only argument tuples holding this value (repeatable: all must match)")
     ap.add_argument("--timeout", type=float, default=1800.0, help="seconds per run")
     ap.add_argument("--numactl", action="store_true", help="bind runs with numactl as the reference does")
     ap.add_argument("--dry-run", action="store_true", help="print the commands only")
@@ -133,6 +138,8 @@ def main(argv=None) -> int:
         if wanted and x.identifier not in wanted:
             continue
         for index, args in enumerate(x.arguments):
+            if any(w not in [str(v) for v in args] for w in a.where):
+                continue
             args = scaled_arguments(x, args, a.scale)
             if a.max_elements is not None and num_keys(x, args) > a.max_elements:
                 print("skip %s %s: above --max-elements" % (x.identifier, args), flush=True)
@@ -155,7 +162,7 @@ def main(argv=None) -> int:
                                timeout=a.timeout)
             rc, text, err = r.returncode, r.stdout, r.stderr
         except subprocess.TimeoutExpired:
-            rc, text, err = None, "", "time limit of %.0f s" % a.timeout
+            rc, text, err = -1, "", "time limit of %.0f s" % a.timeout
         except OSError as ex:
             rc, text, err = None, "", str(ex)
         rows = csv_lines(text, len(x.columns), strict=prof is None) if rc == 0 else None
@@ -166,6 +173,11 @@ def main(argv=None) -> int:
             failed += 1
             tail = (text + err).strip().splitlines()[-5:]
             print("    exit %s; %s" % (rc, " | ".join(tail)), flush=True)
+            if rc is not None and (rc < 0 or rc > 1):
+                # a time limit, a signal or an abort: the device may be in a
+                # bad state, so nothing more runs
+                print("stopping: run ended abnormally (exit %s)" % rc, flush=True)
+                return 3
             continue
         path = out / ("%s_%s.csv" % (x.executable, x.identifier))
         new = not path.exists()

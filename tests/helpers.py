@@ -45,6 +45,13 @@ def random_keys(kt: int, n: int, rng: np.random.Generator, dist: str = "uniform"
         k = ((run << np.uint64(nbits - 16)) | rng.integers(0, 1 << 16, size=n).astype(np.uint64)).astype(bd)
     elif dist == "lowbits":  # only the low byte varies
         k = (rng.integers(0, 256, size=n).astype(np.uint64) | np.uint64(0x1200)).astype(bd)
+    elif dist == "nonneg":  # the top bit of the raw keys clear: non-negative ints and floats, u32 below 2^31
+        k = random_keys(kt, n, rng) & ~bd(1 << (nbits - 1))
+    elif dist == "low28":  # raw keys below 2^28: the top 4 bits of every encoded key agree
+        k = random_keys(kt, n, rng) & bd((1 << 28) - 1)
+    elif dist == "stray":  # low28 with one key of all ones, at a position a sparse sample misses
+        k = random_keys(kt, n, rng, "low28")
+        k[min(1, n - 1)] = bd((1 << nbits) - 1)
     else:
         raise ValueError(dist)
     if kt in (O.K_F32, O.K_F64) and dist in ("uniform", "few") and n >= 16:

@@ -262,15 +262,27 @@ def msd_default_size(n):
 
 def expected_path(kt, keys, forced=True):
     """The three-pass path hands over to the LSD passes when a 16-bit
-    sub-bucket exceeds the 20480 keys of the LDS sort or a byte-3 bucket
-    holds more than twice its share plus two 18432-key tiles.  forced: the
-    path is tried at any size (MGS_MSD_MIN=0)."""
+    sub-bucket exceeds the 20480 keys of the LDS sort, a byte-3 bucket
+    holds more than twice its share plus two 18432-key tiles, or a key
+    differs from the leading bits that a sample of 1024 keys (positions
+    i (n - 1) / 1023) shares.  The sub-buckets are the 16 bits below those
+    shared leading bits (at most 16 of them).  forced: the path is tried at
+    any size (MGS_MSD_MIN=0)."""
     n = keys.size
     if n <= 4096:
         return 1
     if not forced and not msd_default_size(n):
         return 2
-    top = (O.encode(kt, keys) >> np.uint64(16)).astype(np.int64)
+    enc = O.encode(kt, keys).astype(np.uint64)
+    sample = enc[(np.arange(1024, dtype=np.uint64) * np.uint64(n - 1)) // np.uint64(1023)]
+    same, any_ = np.bitwise_and.reduce(sample), np.bitwise_or.reduce(sample)
+    common = 32 - int(same ^ any_).bit_length()
+    low = 16 - min(common, 16)
+    if low < 16:
+        hi_mask = np.uint64((0xFFFFFFFF << (low + 16)) & 0xFFFFFFFF)
+        if np.any((enc & hi_mask) != (same & hi_mask)):
+            return 2
+    top = ((enc >> np.uint64(low)) & np.uint64(0xFFFF)).astype(np.int64)
     sub = np.bincount(top, minlength=1 << 16)
     bucket = sub.reshape(256, 256).sum(axis=1)
     off = sub.max() > 20480 or bucket.max() > 2 * (n // 256) + 2 * 18432
@@ -279,15 +291,19 @@ def expected_path(kt, keys, forced=True):
 
 @pytest.mark.parametrize("kt", MSD_KEYS, ids=lambda k: KEY_NAME[k])
 @pytest.mark.parametrize("n", [4097, 100_003, 1 << 20, (1 << 21) + 777])
-@pytest.mark.parametrize("dist", ["uniform", "sorted", "reverse", "few", "equal", "lowbits", "runs"])
+@pytest.mark.parametrize("dist", ["uniform", "sorted", "reverse", "few", "equal", "lowbits", "runs", "nonneg", "low28",
+                                  "stray"])
 def test_three_pass_path(mg, kt, n, dist, monkeypatch):
     """4-byte keys alone try the three-pass path (MSD pass on byte 3, pass on
     byte 2 inside each bucket, LDS sort of each batch of 16-bit sub-buckets);
     MGS_MSD_MIN=0 tries it at any size.  Keys whose top 16 bits crowd into
     a few sub-buckets (few, equal, lowbits at the larger sizes) hand over to
     the LSD passes.  Runs of keys sharing their top 16 bits (runs, sorted at
-    2^26) switch the passes to the uniform-digit ranking.  Either way the
-    output is the oracle's, in place and out of place."""
+    2^26) switch the passes to the uniform-digit ranking.  Keys whose
+    leading bits agree (nonneg, low28) sort by the 16 bits below them; one
+    key that breaks the agreement a sample saw (stray) hands over to the LSD
+    passes.  Either way the output is the oracle's, in place and out of
+    place."""
     import torch
     monkeypatch.setenv("MGS_MSD_MIN", "0")
     rng = np.random.default_rng(n + 3 * kt)
@@ -307,24 +323,32 @@ def test_three_pass_path(mg, kt, n, dist, monkeypatch):
             assert np.array_equal(from_dev(dk, keys.dtype), keys)
 
 
-@pytest.mark.parametrize("dist,n", [("uniform", (1 << 24) + 12_345), ("sorted", (1 << 24) + 12_345),
-                                    ("reverse", (1 << 24) + 12_345), ("skewed", (1 << 24) + 12_345),
-                                    ("uniform", (1 << 26) + 12_345)])
-def test_three_pass_default_size(mg, dist, n):
+@pytest.mark.parametrize("kt,dist,n", [(O.K_U32, "uniform", (1 << 24) + 12_345), (O.K_U32, "sorted", (1 << 24) + 12_345),
+                                       (O.K_U32, "reverse", (1 << 24) + 12_345), (O.K_U32, "skewed", (1 << 24) + 12_345),
+                                       (O.K_U32, "uniform", (1 << 26) + 12_345), (O.K_U32, "nonneg", (1 << 24) + 12_345),
+                                       (O.K_U32, "low28", (1 << 24) + 12_345), (O.K_U32, "stray", (1 << 24) + 12_345),
+                                       (O.K_I32, "nonneg", (1 << 24) + 12_345), (O.K_F32, "nonneg", (1 << 24) + 12_345)],
+                         ids=lambda v: KEY_NAME.get(v, str(v)) if isinstance(v, int) and v < 8 else str(v))
+def test_three_pass_default_size(mg, kt, dist, n):
     """Without environment the path is tried in [2^23, 3 * 2^24) and from
     2^29 keys (2^26 takes the LSD passes); the output is the oracle's either
-    way."""
+    way.  Non-negative int keys (the generator's int inputs) and keys below
+    2^28 take the three-pass path with the window under their shared leading
+    bits."""
     import torch
-    kt = O.K_U32
     rng = np.random.default_rng(26)
     keys = random_keys(kt, n, rng, dist)
     ek, _ = O.sort(kt, keys)
     temp = mg.temp_buffer(mg.lib.mgs_sort_temp_bytes(kt, mg.VAL_NONE, n), "cuda")
     dk = to_dev(keys)
-    mg.sort("u32", dk, temp=temp)
+    mg.sort(KEY_NAME[kt], dk, temp=temp)
     assert mg.temp_status(temp) == 0
     path = mg.lib.mgs_sort_path(kt, mg.VAL_NONE, n, mg._ptr(temp), temp.numel(), None)
     assert path == expected_path(kt, keys, forced=False)
+    if dist in ("nonneg", "low28") and kt != O.K_F32:  # (the float inputs carry specials that crowd sub-buckets)
+        assert path == 3
+    if dist == "stray":
+        assert path == 2
     assert np.array_equal(from_dev(dk, np.uint32), ek)
 
 
