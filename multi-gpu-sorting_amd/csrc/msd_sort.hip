@@ -62,7 +62,12 @@ constexpr uint32_t kBatchTarget = kLocalCap / 2;
 constexpr uint32_t kNoWork = 0xFFFFFFFFu;
 
 // Decision word bits (MsdCtl::off): any set bit turns the path off.
-enum : uint32_t { kOffOverflow = 1u, kOffSubBucket = 2u, kOffBucket = 4u };
+// kOffSample: a bucket holds more than kSampleCrowd of the 1024 sampled keys
+// (5 times its share: far beyond the scan's limit of twice the share, which
+// uniform keys reach in the sample with a chance of about 1e-6), so the
+// upfront read is skipped and the scan and finish kernels return at once.
+enum : uint32_t { kOffOverflow = 1u, kOffSubBucket = 2u, kOffBucket = 4u, kOffSample = 8u };
+constexpr uint32_t kSampleCrowd = 20;
 
 
 // ---------------------------------------------------------------------------
@@ -82,18 +87,23 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
   __shared__ uint32_t s_rows[kRadix];
   __shared__ uint32_t s_runs, s_stray;
   __shared__ uint32_t s_and[kMsdBlock / kWave], s_or[kMsdBlock / kWave];
+  __shared__ uint32_t s_bh[kRadix];
+  __shared__ uint32_t s_crowd;
   const int tid = threadIdx.x;
   const uint32_t c = blockIdx.x;
   const int lane = tid & (kWave - 1), w = tid / kWave;
   for (int i = tid; i < kMsdBins / 2; i += kMsdBlock) h[i] = 0;
+  if (tid < kRadix) s_bh[tid] = 0;
   if (tid == 0) {
     xcd_of[c] = xcc_id();
     s_stray = 0;
+    s_crowd = 0;
   }
   // the window: every workgroup reads the same sample of 1024 keys, evenly
   // spaced over the input (n >= 2), and derives the same bits from it
+  const uint32_t e_sample = Codec<KT>::enc(keys[((uint64_t)tid * (n - 1)) / (kMsdBlock - 1)]);
   {
-    const uint32_t e = Codec<KT>::enc(keys[((uint64_t)tid * (n - 1)) / (kMsdBlock - 1)]);
+    const uint32_t e = e_sample;
     uint32_t ea = e, eo = e;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -116,6 +126,16 @@ __global__ __launch_bounds__(kMsdBlock) void msd_upfront_kernel(const uint32_t* 
   const uint32_t low = 16u - min(common, 16u);
   const uint32_t hi_mask = low < 16u ? ~0u << (low + 16u) : 0u;  // the shared bits above the window
   const uint32_t hi_val = s_a & hi_mask;
+  // the sample's keys per bucket (the window's high byte): a crowded bucket
+  // hands the sort to the LSD passes before the read
+  atomicAdd(&s_bh[(e_sample >> (low + 8u)) & 0xFFu], 1u);
+  __syncthreads();
+  if (tid < kRadix && s_bh[tid] > kSampleCrowd) s_crowd = 1u;
+  __syncthreads();
+  if (s_crowd) {
+    if (c == 0 && tid == 0) ctl->off = kOffSample;  // (the LSD sort clears the status word)
+    return;
+  }
   if (c == 0 && tid == 0) {  // nothing else writes these during this kernel
     *status = 0;
     ctl->off = 0;
@@ -240,6 +260,7 @@ __global__ __launch_bounds__(256) void msd_scan_kernel(const uint32_t* __restric
                                                        uint32_t* __restrict__ batch_cnt,
                                                        const uint32_t* __restrict__ chain_bad,
                                                        const uint32_t* __restrict__ xcd_of) {
+  if (ctl->off & kOffSample) return;
   __shared__ uint32_t sums[4];
   __shared__ uint32_t s_carry;
   const uint32_t b = blockIdx.x, d = threadIdx.x;
@@ -328,6 +349,7 @@ __global__ __launch_bounds__(256) void msd_finish_kernel(MsdCtl* __restrict__ ct
                                                          uint32_t* __restrict__ tile_base1,
                                                          uint32_t* __restrict__ tile_base2,
                                                          uint32_t* __restrict__ counters) {
+  if (ctl->off & kOffSample) return;
   __shared__ uint32_t sums[4];
   const uint32_t d = threadIdx.x;
   const uint32_t bt = bucket_tot[d];

@@ -49,6 +49,11 @@ def random_keys(kt: int, n: int, rng: np.random.Generator, dist: str = "uniform"
         k = random_keys(kt, n, rng) & ~bd(1 << (nbits - 1))
     elif dist == "low28":  # raw keys below 2^28: the top 4 bits of every encoded key agree
         k = random_keys(kt, n, rng) & bd((1 << 28) - 1)
+    elif dist == "crowd":  # a third of the keys share their top byte (0x5A), the rest uniform
+        k = random_keys(kt, n, rng).astype(np.uint64)
+        m = rng.random(n) < 1 / 3
+        k[m] = (k[m] & np.uint64((1 << (nbits - 8)) - 1)) | np.uint64(0x5A << (nbits - 8))
+        k = k.astype(bd)
     elif dist == "stray":  # low28 with one key of all ones, at a position a sparse sample misses
         k = random_keys(kt, n, rng, "low28")
         k[min(1, n - 1)] = bd((1 << nbits) - 1)

@@ -265,7 +265,8 @@ def expected_path(kt, keys, forced=True):
     sub-bucket exceeds the 20480 keys of the LDS sort, a byte-3 bucket
     holds more than twice its share plus two 18432-key tiles, or a key
     differs from the leading bits that a sample of 1024 keys (positions
-    i (n - 1) / 1023) shares.  The sub-buckets are the 16 bits below those
+    i (n - 1) / 1023) shares, or at once when a bucket holds more than 20
+    of the sampled keys.  The sub-buckets are the 16 bits below those
     shared leading bits (at most 16 of them).  forced: the path is tried at
     any size (MGS_MSD_MIN=0)."""
     n = keys.size
@@ -278,6 +279,10 @@ def expected_path(kt, keys, forced=True):
     same, any_ = np.bitwise_and.reduce(sample), np.bitwise_or.reduce(sample)
     common = 32 - int(same ^ any_).bit_length()
     low = 16 - min(common, 16)
+    # a bucket (the window's high byte) holding more than 20 sampled keys
+    # hands over before the read
+    if np.bincount(((sample >> np.uint64(low + 8)) & np.uint64(0xFF)).astype(np.int64), minlength=256).max() > 20:
+        return 2
     if low < 16:
         hi_mask = np.uint64((0xFFFFFFFF << (low + 16)) & 0xFFFFFFFF)
         if np.any((enc & hi_mask) != (same & hi_mask)):
@@ -324,6 +329,7 @@ def test_three_pass_path(mg, kt, n, dist, monkeypatch):
 
 
 @pytest.mark.parametrize("kt,dist,n", [(O.K_U32, "uniform", (1 << 24) + 12_345), (O.K_U32, "sorted", (1 << 24) + 12_345),
+                                       (O.K_U32, "few", (1 << 24) + 12_345), (O.K_U32, "crowd", (1 << 24) + 12_345),
                                        (O.K_U32, "reverse", (1 << 24) + 12_345), (O.K_U32, "skewed", (1 << 24) + 12_345),
                                        (O.K_U32, "uniform", (1 << 26) + 12_345), (O.K_U32, "nonneg", (1 << 24) + 12_345),
                                        (O.K_U32, "low28", (1 << 24) + 12_345), (O.K_U32, "stray", (1 << 24) + 12_345),
