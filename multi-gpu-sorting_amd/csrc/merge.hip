@@ -1,8 +1,10 @@
 // Merge path, Tanasic pivot, range swap and filler kernels for gfx950.
 //
-// Merge: Green, McColl & Bader, "GPU Merge Path" (2012).  A partition kernel
-// finds, for every output tile boundary (diagonal), how many elements of A
-// precede it; each workgroup then stages its A and B slices in LDS, every
+// Merge: Green, McColl & Bader, "GPU Merge Path" (2012).  Two partition
+// kernels find, for every output tile boundary (diagonal), how many elements
+// of A precede it (every 16th boundary over the whole inputs first, then the
+// others inside the bracket of their coarse neighbours); each workgroup then
+// stages its A and B slices in LDS, every
 // thread co-ranks its own run of outputs inside the tile, merges it serially
 // and the tile is written back with consecutive lanes on consecutive
 // addresses.  Ties go to A (stable), as thrust::merge does at reference
@@ -27,11 +29,22 @@ constexpr int kMergeIpt = MGS_MERGE_IPT;  // odd: staggers LDS banks of the seri
 constexpr int kMergeTile = kMergeBlock * kMergeIpt;
 static_assert(kMergeIpt % 2 == 1 && kMergeTile >= 3840, "odd items per thread; tiles of at least 3840 keys");
 
-// Number of A elements among the first `diag` outputs of the stable merge.
+// Tile boundaries per coarse split: the partition first co-ranks every
+// kCoarse-th boundary over the whole inputs, then every boundary inside the
+// bracket its two coarse neighbours give (the split is monotone in the
+// diagonal for sorted inputs): about half the dependent global loads, and
+// the fine searches of one bracket share cache lines.
+constexpr uint64_t kCoarse = 16;
+
+// Number of A elements among the first `diag` outputs of the stable merge,
+// searched inside [lo_bound, hi_bound] (pass 0 and na for no bracket).
 template <int KT, typename Bits>
-__device__ __forceinline__ uint64_t corank(const Bits* a, uint64_t na, const Bits* b, uint64_t nb, uint64_t diag) {
+__device__ __forceinline__ uint64_t corank(const Bits* a, uint64_t na, const Bits* b, uint64_t nb, uint64_t diag,
+                                           uint64_t lo_bound = 0, uint64_t hi_bound = ~0ull) {
   uint64_t lo = diag > nb ? diag - nb : 0;
   uint64_t hi = diag < na ? diag : na;
+  lo = std::max(lo, std::min(lo_bound, hi));
+  hi = std::max(lo, std::min(hi, hi_bound));
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
     // A[mid] is taken before B[diag-mid-1] iff enc(A[mid]) <= enc(B[diag-mid-1])
@@ -43,14 +56,30 @@ __device__ __forceinline__ uint64_t corank(const Bits* a, uint64_t na, const Bit
   return lo;
 }
 
+// coarse[q]: the split of tile boundary q kCoarse, for q <= num_coarse.
+template <int KT>
+__global__ void merge_coarse_kernel(const typename Codec<KT>::Bits* a, uint64_t na,
+                                    const typename Codec<KT>::Bits* b, uint64_t nb, uint64_t* coarse,
+                                    uint64_t num_coarse) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > num_coarse) return;
+  const uint64_t diag = std::min<uint64_t>(q * kCoarse * kMergeTile, na + nb);
+  coarse[q] = corank<KT>(a, na, b, nb, diag);
+}
+
+// splits[i]: the split of tile boundary i, searched between the coarse
+// splits around it.
 template <int KT>
 __global__ void merge_partition_kernel(const typename Codec<KT>::Bits* a, uint64_t na,
                                        const typename Codec<KT>::Bits* b, uint64_t nb, uint64_t* splits,
-                                       uint64_t num_tiles) {
+                                       uint64_t num_tiles, const uint64_t* __restrict__ coarse,
+                                       uint64_t num_coarse) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i > num_tiles) return;
   const uint64_t diag = std::min<uint64_t>(i * kMergeTile, na + nb);
-  splits[i] = corank<KT>(a, na, b, nb, diag);
+  const uint64_t q = i / kCoarse;
+  const uint64_t lo = coarse[q], hi = coarse[std::min(q + 1, num_coarse)];
+  splits[i] = corank<KT>(a, na, b, nb, diag, lo, std::max(lo, hi));
 }
 
 template <int KT, typename V>
@@ -179,11 +208,15 @@ int merge_typed(const void* ak, const void* av, size_t na, const void* bk, const
   const size_t total = na + nb;
   if (total == 0) return MGS_OK;
   const uint64_t tiles = (total + kMergeTile - 1) / kMergeTile;
-  if (temp == nullptr || temp_bytes < (tiles + 1) * sizeof(uint64_t)) return MGS_ERR_TEMP_TOO_SMALL;
+  const uint64_t num_coarse = (tiles + kCoarse - 1) / kCoarse;
+  if (temp == nullptr || temp_bytes < (tiles + 1 + num_coarse + 1) * sizeof(uint64_t)) return MGS_ERR_TEMP_TOO_SMALL;
   uint64_t* splits = static_cast<uint64_t*>(temp);
+  uint64_t* coarse = splits + tiles + 1;
   const Bits* a = static_cast<const Bits*>(ak);
   const Bits* b = static_cast<const Bits*>(bk);
-  merge_partition_kernel<KT><<<(unsigned)((tiles + 1 + 255) / 256), 256, 0, s>>>(a, na, b, nb, splits, tiles);
+  merge_coarse_kernel<KT><<<(unsigned)((num_coarse + 1 + 255) / 256), 256, 0, s>>>(a, na, b, nb, coarse, num_coarse);
+  merge_partition_kernel<KT><<<(unsigned)((tiles + 1 + 255) / 256), 256, 0, s>>>(a, na, b, nb, splits, tiles, coarse,
+                                                                                   num_coarse);
   merge_tile_kernel<KT, V><<<(unsigned)tiles, kMergeBlock, 0, s>>>(
       a, static_cast<const V*>(av), na, b, static_cast<const V*>(bv), nb, static_cast<Bits*>(ok),
       static_cast<V*>(ov), splits);
@@ -289,7 +322,7 @@ int swap_bytes(void* a, void* b, size_t n, int bytes, hipStream_t s) {
 size_t merge_temp_bytes(mgs_key kt, mgs_val vt, size_t na, size_t nb) {
   if (key_bytes(kt) < 0 || val_bytes(vt) < 0) return 0;
   const size_t tiles = (na + nb + kMergeTile - 1) / kMergeTile;
-  return (tiles + 1) * sizeof(uint64_t);
+  return (tiles + 1 + (tiles + kCoarse - 1) / kCoarse + 1) * sizeof(uint64_t);
 }
 
 int merge_dispatch(mgs_key kt, mgs_val vt, const void* ak, const void* av, size_t na, const void* bk, const void* bv,
