@@ -59,6 +59,7 @@ __global__ void init_kernel(uint32_t* __restrict__ hist, uint32_t words, uint32_
     red[0] = 0;     // OR identity
     red[1] = ~0ull;  // AND identity
     red[2] = 0;      // XCDs seen by the upfront kernel
+    red[3] = 0;      // crowded bytes of the upfront kernel's sample
   }
 }
 
@@ -84,20 +85,51 @@ __device__ __forceinline__ void block_chain(uint32_t b, uint32_t grid, uint32_t 
   nb = (grid - c + chains - 1) / chains;
 }
 
+// Keys the upfront kernel samples, and the count of one digit above which
+// a byte is crowded.
+constexpr int kCrowdSamples = 1024;
+constexpr uint32_t kCrowdSample = kCrowdSamples / 16;
+
 // Byte-0 histogram of one chain per workgroup plus OR/AND of the encoded
 // keys; also clears the look-back descriptors (desc_words 32-bit words).
 // Workgroups are split chain-major, so each flushes one 256-bin histogram.
+// Workgroup 0 also reads 1024 keys evenly spaced over the input and sets bit
+// b of red[3] when one digit of byte b holds more than kCrowdSample of them
+// (a sixteenth): the pass before byte b then takes its uniform-digit launch
+// (RANK 2), which counts byte b with one atomic per digit group, since one
+// atomic per key would put most lanes of a wavefront on one LDS counter in
+// turn.
 template <int KT, int BLOCK, bool VEC>
 __global__ __launch_bounds__(BLOCK) void upfront_kernel(const typename Codec<KT>::Bits* __restrict__ keys, uint32_t n,
                                                         uint32_t* __restrict__ hist, uint64_t* __restrict__ red,
                                                         uint32_t* __restrict__ desc, size_t desc_words, ChainMap cm,
                                                         int record_xcd, const uint32_t* __restrict__ gate) {
   using Bits = typename Codec<KT>::Bits;
+  constexpr int P = (int)sizeof(Bits);
   if (gated_off(gate)) return;
   __shared__ uint32_t h[kRadix];
   __shared__ Bits s_or[BLOCK / kWave], s_and[BLOCK / kWave];
   const int tid = threadIdx.x;
   for (int i = tid; i < kRadix; i += BLOCK) h[i] = 0;
+  if (blockIdx.x == 0) {
+    __shared__ uint32_t sh[P][kRadix];
+    __shared__ uint32_t s_mask;
+    for (int i = tid; i < P * kRadix; i += BLOCK) (&sh[0][0])[i] = 0;
+    if (tid == 0) s_mask = 0;
+    __syncthreads();
+    for (int s = tid; s < kCrowdSamples; s += BLOCK) {
+      const Bits e = Codec<KT>::enc(keys[((uint64_t)s * (n - 1)) / (kCrowdSamples - 1)]);
+#pragma unroll
+      for (int b = 0; b < P; ++b) atomicAdd(&sh[b][(uint32_t)(e >> (8 * b)) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    uint32_t m = 0;
+    for (int i = tid; i < P * kRadix; i += BLOCK)
+      if ((&sh[0][0])[i] > kCrowdSample) m |= 1u << (i / kRadix);
+    if (m) atomicOr(&s_mask, m);
+    __syncthreads();
+    if (tid == 0 && s_mask) atomicOr(reinterpret_cast<unsigned long long*>(&red[3]), (unsigned long long)s_mask);
+  }
   {
     const size_t gtid = (size_t)blockIdx.x * BLOCK + tid;
     const size_t gstride = (size_t)gridDim.x * BLOCK;
@@ -405,6 +437,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
   const int na = __builtin_popcount(vary);
   const uint32_t above = vary & ~((2u << pass) - 1u);
   const int next = above ? __builtin_ctz(above) : -1;
+  const bool next_crowded = next >= 0 && ((red[3] >> next) & 1u) != 0;
   const int sb = src_of(k, na, aliased), db = dst_of(k, na, aliased);
   const Bits* __restrict__ src = pick_src(sb, k_in, (const Bits*)k_alt, (const Bits*)k_out);
   Bits* __restrict__ dst = pick_dst(db, k_alt, k_out);
@@ -463,7 +496,9 @@ __global__ __launch_bounds__(BLOCK, WAVES) void pass_kernel(
     }
     dbase = block_excl_scan_256(tot, sums, tid);  // contains a barrier
     if constexpr (RANK == 1 || RANK == 2) {
-      if (tid < kRadix && tot > (n >> MGS_SKEW_SHIFT)) s_skew = 1;
+      // a crowded next byte also takes the uniform-digit launch, whose
+      // write-out counts that byte with one atomic per digit group
+      if ((tid < kRadix && tot > (n >> MGS_SKEW_SHIFT)) || (tid == 0 && next_crowded)) s_skew = 1;
     }
     if (tid < kRadix)
       for (uint32_t c = 0; c < (uint32_t)kChains; ++c) nh[c][tid] = 0;
@@ -808,7 +843,7 @@ SortLayout sort_layout_cfg(size_t n) {
   L.alt_k = take(n * KB);
   L.alt_v = take(n * VB);
   L.hist = take((size_t)KB * kChains * kRadix * 4);
-  L.red = take(3 * sizeof(uint64_t));
+  L.red = take(4 * sizeof(uint64_t));
   L.counters = take((size_t)KB * L.rounds * kChains * 4);
   L.carry = take(2 * (size_t)kChains * kRadix * 4);
   L.msd_ctl = take(sizeof(MsdCtl));  // the three-pass path's decision word: read by the LSD kernels
