@@ -32,7 +32,7 @@ namespace mgs {
 
 namespace {
 
-constexpr int kMwBlock = 512;
+constexpr int kMwBlock = 1024;
 constexpr uint32_t kMwCap = 8192;  // keys per piece at most
 // sorted samples per piece: 3k (at least 8), so that a piece holds on
 // average step / (step + k) = 3/4 of the keys its LDS bound allows
