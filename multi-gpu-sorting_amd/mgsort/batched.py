@@ -164,6 +164,11 @@ def run(dist: str = "uniform", nseg: int = 4096, seed: int = 2024, reps: int = 3
         tn = tuning.as_tuning(point)
         tuned = {"point": point, "source": f"timed on this batch ({len(timings)} points)",
                  "ms": round(best_ms, 3), "default_ms": round(default_ms, 3)}
+    # the longest array bounds the segment length: the large-segment passes
+    # then use 32-bit look-back descriptors although the batch holds more
+    # than 2^30 keys
+    tn.max_segment_keys = int(lengths.max())
+    tuned["max_segment_keys"] = tn.max_segment_keys
     blk, ipt = tuning.geometry(KEY_U64, VAL_U32, tn.large_variant)
     tuned["geometry"] = {"block": blk, "items_per_thread": ipt}
     need = lib.mgs_segmented_sort_temp_bytes_tuned(KEY_U64, VAL_U32, total, nseg, ctypes.byref(tn))
