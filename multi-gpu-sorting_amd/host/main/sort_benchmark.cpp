@@ -73,7 +73,9 @@ struct Options {
 };
 
 // The "thrust" branch (reference :81-91): H2D, sort, D2H inside sort_phase.
-// The "hip" branch: in-HBM sort, copies outside the timed phase.
+// The "hip" branch: in-HBM sort, copies outside the timed phase, after one
+// untimed sort of a small scratch array on the same stream (the first
+// kernel launch loads the library's code objects).
 template <typename T>
 void DeviceSort(HostVector<T>* e, const Options& o, bool include_copies) {
   HIP_CHECK(hipSetDevice(o.gpus.front()));
@@ -96,7 +98,16 @@ void DeviceSort(HostVector<T>* e, const Options& o, bool include_copies) {
       HIP_CHECK(hipMemcpy(vals.get(), idx.data(), n * 8, hipMemcpyHostToDevice));
     }
   }
-  if (!include_copies) HIP_CHECK(hipMemcpy(keys.get(), e->data(), n * sizeof(T), hipMemcpyHostToDevice));
+  if (!include_copies) {
+    HIP_CHECK(hipMemcpy(keys.get(), e->data(), n * sizeof(T), hipMemcpyHostToDevice));
+    const size_t wn = std::min<size_t>(n, 65536);
+    if (wn > 0) {
+      DeviceBuffer warm(wn * sizeof(T));
+      HIP_CHECK(hipMemcpy(warm.get(), e->data(), wn * sizeof(T), hipMemcpyHostToDevice));
+      MGS_CHECK(mgs_sort(kt, MGS_VAL_NONE, warm.get(), warm.get(), nullptr, nullptr, wn, temp.get(), temp.bytes(), s));
+      s.Sync();
+    }
+  }
   TimeDurations::Get()->Tic("sort_phase");
   if (include_copies)
     HIP_CHECK(hipMemcpyAsync(keys.get(), e->data(), n * sizeof(T), hipMemcpyHostToDevice, s));
