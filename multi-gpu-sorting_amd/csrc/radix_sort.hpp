@@ -207,39 +207,19 @@ __device__ __forceinline__ uint32_t rank_wave(uint32_t d, uint64_t* peers, uint3
 //                  per pass from the digit counts (uniform-digit path when
 //                  one digit holds more than n >> MGS_SKEW_SHIFT keys,
 //                  1/64 of them by default)
-//   MGS_BATCH_WO   write-out of full tiles in batches of 8 items
-//   MGS_SPLIT_LB   1: look-back loads on waves 4..7, descriptor stores on
-//                  0..3; 2: waves 4..7 own the descriptors and finish the
-//                  look-back after the scatter into LDS
 //   MGS_XCD_LOCAL  chains served per XCD with workgroup-scope descriptor
 //                  stores (when the device shows eight XCDs)
-//   MGS_LB_STEP    (with MGS_SPLIT_LB 2) one look-back round taken before
-//                  the scatter
-//   MGS_EARLY_PF   (with MGS_SPLIT_LB 2) next tile taken during the digit
-//                  scan and loaded right after each wavefront's scatter
-#ifndef MGS_LB_STEP
-#define MGS_LB_STEP 1
-#endif
 #ifndef MGS_XCD_LOCAL
 #define MGS_XCD_LOCAL 1
 #endif
-#ifndef MGS_EARLY_PF
-#define MGS_EARLY_PF 1
-#endif
 #ifndef MGS_UNI
 #define MGS_UNI 0
-#endif
-#ifndef MGS_BATCH_WO
-#define MGS_BATCH_WO 0
 #endif
 #ifndef MGS_SKEW_PAIR_MIN
 #define MGS_SKEW_PAIR_MIN (1u << 26)
 #endif
 #ifndef MGS_SKEW_SHIFT
 #define MGS_SKEW_SHIFT 6
-#endif
-#ifndef MGS_SPLIT_LB
-#define MGS_SPLIT_LB 2
 #endif
 
 // With UNI (the uniform-digit path), the lanes that carry the first active
