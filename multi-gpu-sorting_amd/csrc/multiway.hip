@@ -187,16 +187,15 @@ __device__ __forceinline__ uint32_t corank_lds(const Bits* a, uint32_t na, const
 // One piece: its slices of the k runs are loaded into LDS back to back (run
 // order), then merged in ceil(log2 k) rounds, each merging adjacent slice
 // pairs (2i, 2i+1) into the other LDS buffer (earlier runs first on ties, so
-// the order stays stable).  In a round every thread writes kSlots
-// consecutive outputs: one co-rank search in its pair, then a serial merge.
+// the order stays stable).  In a round every thread writes q consecutive
+// outputs (q odd, q * kMwBlock >= the piece's keys, q <= kCap / kMwBlock + 1):
+// one co-rank search in its pair, then a serial merge.
 template <int KT, typename Bits>
 __global__ __launch_bounds__(kMwBlock) void mw_piece_kernel(const Bits* const* __restrict__ runs, uint32_t k,
                                                             const uint64_t* __restrict__ splits,
                                                             const uint64_t* __restrict__ bases, uint64_t first_piece,
                                                             Bits* __restrict__ out, uint32_t* __restrict__ status) {
-  // outputs per thread in a merge round: odd, so that the lanes' serial
-  // merges start in different LDS banks; the loads use kLoads per thread
-  constexpr uint32_t kSlots = kMwCap / kMwBlock + 1;
+  // loads per thread: the piece's keys at most
   constexpr uint32_t kLoads = kMwCap / kMwBlock;
   __shared__ Bits buf[2][kMwCap];
   __shared__ uint32_t bnd[kMwMaxRuns + 1];
@@ -232,6 +231,12 @@ __global__ __launch_bounds__(kMwBlock) void mw_piece_kernel(const Bits* const* _
     return;
   }
   const uint32_t cnt = bnd[k];
+  // outputs per thread in a merge round, odd so that the lanes' serial
+  // merges start in different LDS banks: with several rounds the fewest that
+  // cover the piece (a piece of 6144 keys, the average, takes 7 per thread);
+  // a single round (k <= 2) keeps kCap / kMwBlock + 1, which leaves fewer
+  // co-rank searches (measured faster there, slower with more rounds)
+  const uint32_t q = k <= 2 ? kMwCap / kMwBlock + 1 : ((cnt + kMwBlock - 1) / kMwBlock) | 1u;
   {
     // every load of the piece issued before the first LDS store
     Bits v[kLoads];
@@ -263,8 +268,8 @@ __global__ __launch_bounds__(kMwBlock) void mw_piece_kernel(const Bits* const* _
     const Bits* src = buf[cur];
     Bits* dst = buf[cur ^ 1];
     const uint32_t pairs = (slices + 1) >> 1;
-    uint32_t o = (uint32_t)tid * kSlots;
-    const uint32_t end = min(o + kSlots, cnt);
+    uint32_t o = (uint32_t)tid * q;
+    const uint32_t end = min(o + q, cnt);
     while (o < end) {
       // the last pair i with bnd[2i] <= o: it ends beyond o (a later pair
       // starts beyond o, or it is the last pair, which ends at cnt)
